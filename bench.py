@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the accelerated nalgebra path on one MI355X.
+
+Default workload (the batched-Cholesky half of the headline metric):
+65,536 f64 SPD systems, n_j uniform integer in [4, 64], A_j = G G^T + n_j I
+with G ~ N(0,1), 4 right-hand sides ~ N(0,1); every member is factored
+(Cholesky::new) and solved (Cholesky::solve_mut) in one hipGraph replay of
+the batched kernels.  A step is one such pass over a fresh, device-resident
+copy of the batch (the factorisation is in place, so each timed step gets its
+own copy, prepared before the timed region).
+
+    python bench.py [--steps K] [--warmup W] [--mode exact|fast]
+                    [--workload chol|lu] [--no-cpu]
+
+Prints one JSON line.  `value` is matrices/s; `roofline` prices the step
+against HBM (algorithmic bytes: read the lower triangle of A and B, write L
+and X); `cpu_baseline` times the CPU oracle (a C restatement of nalgebra's
+Cholesky::new + solve_mut, one thread) on the same batch.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "n=8192 LU-solve time, mixed+IR vs FP64 at equal error; batch Cholesky mats/s"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--mode", choices=["exact", "fast"], default="exact")
+    p.add_argument("--workload", choices=["chol", "lu"], default="chol")
+    p.add_argument("--count", type=int, default=65536)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    return p.parse_args()
+
+
+def draw_sizes(lib, seed, count, lo=4, hi=64):
+    u = np.zeros(count)
+    lib.nrocm_gen_uniform(seed, count, 0.0, 1.0, u.ctypes.data_as(C.c_void_p))
+    return np.minimum(lo + np.floor(u * (hi - lo + 1)), hi).astype(np.int32)
+
+
+def gen_spd(lib, seed, ns, nrhs):
+    n64 = ns.astype(np.int64)
+    offA = np.concatenate([[0], np.cumsum(n64 * n64)[:-1]]).astype(np.int64)
+    offB = np.concatenate([[0], np.cumsum(n64 * nrhs)[:-1]]).astype(np.int64)
+    A = np.zeros(int((n64 * n64).sum()))
+    B = np.zeros(int((n64 * nrhs).sum()))
+    lib.nrocm_gen_spd_batch(seed, len(ns), ns.ctypes.data_as(C.POINTER(C.c_int)), nrhs,
+                            A.ctypes.data_as(C.c_void_p), offA.ctypes.data_as(C.c_void_p),
+                            B.ctypes.data_as(C.c_void_p), offB.ctypes.data_as(C.c_void_p))
+    return A, offA, B, offB
+
+
+def cpu_baseline_chol(ns, A, offA, B, offB, nrhs, budget_s):
+    """Time the oracle's per-matrix Cholesky::new + solve_mut (one thread)."""
+    from oracle import oracle as orc
+    done = 0
+    t_total = 0.0
+    reps = 0
+    while t_total < budget_s and reps < 20:
+        Aw, Bw = A.copy(), B.copy()
+        t0 = time.perf_counter()
+        orc.potrf_potrs_batched(ns, Aw, offA, Bw, offB, nrhs)
+        t_total += time.perf_counter() - t0
+        done += len(ns)
+        reps += 1
+    return {"value": done / t_total, "unit": "matrices/s", "cores": 1, "kind": "port",
+            "sample": f"the full {len(ns)}-matrix batch x{reps} (Cholesky::new + solve_mut per matrix), "
+                      f"{t_total:.1f} s of CPU time"}
+
+
+def bench_chol(args):
+    import torch
+    import nalgebra_amd as nl
+    from nalgebra_amd import _lib
+
+    lib = _lib.lib()
+    nrhs = 4
+    ns = draw_sizes(lib, args.seed, args.count)
+    A, offA, B, offB = gen_spd(lib, args.seed, ns, nrhs)
+    n64 = ns.astype(np.int64)
+    alg_bytes = int((8 * (n64 * (n64 + 1) + 2 * nrhs * n64)).sum())
+    alg_flops = float((n64 ** 3 / 3.0 + 2.0 * nrhs * n64 ** 2).sum())
+    mode = nl.EXACT if args.mode == "exact" else nl.FAST
+
+    dev = torch.device("cuda:0")
+    A0 = torch.from_numpy(A).to(dev)
+    B0 = torch.from_numpy(B).to(dev)
+    K = max(1, args.steps)
+    copies = [(A0.clone(), B0.clone()) for _ in range(K)]
+    bt = nl.CholeskyBatch(ns, nrhs, device=dev)
+    ptrs = [bt.pointers(a, b) for a, b in copies]
+    infos = [torch.empty(len(ns), dtype=torch.int32, device=dev) for _ in range(K)]
+
+    # warm-up: capture one graph per copy, then restore the inputs
+    for i in range(max(args.warmup, K)):
+        j = i % K
+        bt.solve_ptrs(ptrs[j][0], ptrs[j][1], infos[j], mode)
+    torch.cuda.synchronize()
+    for a, b in copies:
+        a.copy_(A0)
+        b.copy_(B0)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream(dev)
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        ev0[i].record(stream)
+        bt.solve_ptrs(ptrs[i][0], ptrs[i][1], infos[i], mode)
+        ev1[i].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3 / K
+    step_ms = [ev0[i].elapsed_time(ev1[i]) for i in range(K)]
+    kern_ms = float(np.mean(step_ms))
+
+    bad = sum(int(x.count_nonzero().item()) for x in infos)
+    if bad:
+        raise SystemExit(f"{bad} members reported a non-positive pivot")
+    # spot-check the last step against the oracle on a sample of members
+    from oracle import oracle as orc
+    Bg = copies[-1][1].cpu().numpy()
+    sample = np.arange(0, len(ns), max(1, len(ns) // 512))
+    Ar, Br = A.copy(), B.copy()
+    sub_ns = ns[sample]
+    orc.potrf_potrs_batched(sub_ns, Ar, offA[sample], Br, offB[sample], nrhs)
+    max_rel = 0.0
+    bit_equal = True
+    for j in sample:
+        sl = slice(offB[j], offB[j] + nrhs * ns[j])
+        g, r = Bg[sl], Br[sl]
+        max_rel = max(max_rel, float(np.linalg.norm(g - r) / np.linalg.norm(r)))
+        bit_equal &= bool(np.array_equal(g.view(np.uint64), r.view(np.uint64)))
+
+    out = {
+        "metric": METRIC,
+        "value": len(ns) / (wall_ms * 1e-3),
+        "unit": "matrices/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched_cholesky_solve", "count": int(len(ns)), "n_range": [4, 64],
+                   "nrhs": nrhs, "mode": args.mode, "seed": args.seed, "graph": True},
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg_bytes / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_step": alg_bytes},
+        "gflops": alg_flops / (kern_ms * 1e-3) / 1e9,
+        "gpu_event_ms_per_step": kern_ms,
+        "parity": {"sampled_members": int(len(sample)), "max_rel_err": max_rel, "bit_identical": bit_equal},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_chol(ns, A, offA, B, offB, nrhs, args.cpu_seconds)
+    return out
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (batched members shard "
+              "across GPUs without communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "chol":
+        out = bench_chol(args)
+    else:
+        from nalgebra_amd.bench_lu import bench_lu
+        out = bench_lu(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

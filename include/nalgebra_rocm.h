@@ -1,0 +1,192 @@
+/*
+ * nalgebra_rocm.h -- C ABI of the MI355X (gfx950) implementation of
+ * nalgebra's dense hot path: gemm, LU with partial pivoting + solve,
+ * Cholesky + solve, Householder QR (+ tall least squares), plus mixed
+ * precision LU with iterative refinement, batched uneven Cholesky and the
+ * adjoint of the Cholesky solve.
+ *
+ * Conventions
+ *  - Matrices are nalgebra DMatrix storage: column-major, element (i, j) at
+ *    ptr[i + j*ld] (VecStorage, src/base/vec_storage.rs:26-32,237-239).
+ *  - Every matrix, vector, pivot and info pointer is a DEVICE pointer, unless
+ *    a parameter's name ends in _host.  Work is enqueued on the handle's
+ *    stream and is asynchronous with respect to the host.
+ *  - Return value: NROCM_OK (0), a negative value -k when argument k
+ *    (1-based, the handle being argument 1) is invalid -- nalgebra panics in
+ *    those cases -- or a positive NROCM_ERR_* for runtime failures.
+ *    Numerical outcomes (singular U, non-positive-definite pivot) are
+ *    reported through device-side info words, as nalgebra returns them as
+ *    values (bool / Option) and LAPACK through info.
+ *  - mode NROCM_EXACT reproduces nalgebra's arithmetic bit for bit (same
+ *    per-element operation order, no fused multiply-add); NROCM_FAST allows
+ *    fused multiply-adds and MFMA blocking (results within 1e-12 relative,
+ *    LU pivots may differ only at near-ties).
+ */
+#ifndef NALGEBRA_ROCM_H
+#define NALGEBRA_ROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NROCM_OK 0
+#define NROCM_ERR_HIP 1          /* a HIP runtime call failed */
+#define NROCM_ERR_ALLOC 2        /* device allocation failed */
+#define NROCM_ERR_UNSUPPORTED 3  /* valid request this build does not implement */
+#define NROCM_ERR_NOT_CONVERGED 4 /* iterative refinement missed its tolerance */
+
+#define NROCM_EXACT 0
+#define NROCM_FAST 1
+
+#define NROCM_PREC_FP64 0
+#define NROCM_PREC_FP32 1
+#define NROCM_PREC_BF16 2
+
+typedef struct nrocm_handle_s *nrocm_handle;
+typedef struct nrocm_batch_s *nrocm_batch;
+
+/* ---------------------------------------------------------------- runtime
+ * No reference counterpart (nalgebra has no device state); the handle owns
+ * the stream binding, workspaces, graph cache and tuning table. */
+int nrocm_create(nrocm_handle *handle);
+int nrocm_destroy(nrocm_handle handle);
+int nrocm_set_stream(nrocm_handle handle, void *hip_stream);
+int nrocm_get_stream(nrocm_handle handle, void **hip_stream);
+const char *nrocm_status_string(int status);
+const char *nrocm_last_error(nrocm_handle handle);
+int nrocm_version(void);
+
+/* ---------------------------------------------------------------- gemm (a1)
+ * C = alpha*A*B + beta*C with A m x k, B k x n, C m x n and arbitrary
+ * element strides (row stride rs*, column stride cs*).  beta == 0 => C is not
+ * read.  Replaces matrixmultiply::dgemm / sgemm as called from
+ * src/base/blas_uninit.rs:276-313 (same argument order), i.e. Matrix::gemm
+ * (src/base/blas.rs:729) and &A * &B (src/base/ops.rs:565-571). */
+int nrocm_dgemm(nrocm_handle handle, size_t m, size_t k, size_t n, double alpha,
+                const double *a, ptrdiff_t rsa, ptrdiff_t csa,
+                const double *b, ptrdiff_t rsb, ptrdiff_t csb, double beta,
+                double *c, ptrdiff_t rsc, ptrdiff_t csc);
+int nrocm_sgemm(nrocm_handle handle, size_t m, size_t k, size_t n, float alpha,
+                const float *a, ptrdiff_t rsa, ptrdiff_t csa,
+                const float *b, ptrdiff_t rsb, ptrdiff_t csb, float beta,
+                float *c, ptrdiff_t rsc, ptrdiff_t csc);
+
+/* ---------------------------------------------------------------- LU (a4-a7)
+ * In-place LU with partial pivoting, packed unit-L \ U exactly as LU::new
+ * (src/linalg/lu.rs:93-122, gauss_step_swap :362-389) leaves it.
+ * ipiv[i] (int32, 0-based) = pivot row chosen at step i (i when no swap;
+ * LU's PermutationSequence holds the pairs (i, ipiv[i]) with ipiv[i] != i,
+ * src/linalg/permutation_sequence.rs:84-93).  info[0] = 0, or 1 + the first
+ * step whose column was entirely zero (nalgebra skips it, lu.rs:107-110).
+ * Shape of nalgebra-lapack's LUScalar::xgetrf (nalgebra-lapack/src/lu.rs:351-380). */
+int nrocm_dgetrf(nrocm_handle handle, int m, int n, double *a, int lda, int *ipiv,
+                 int *info, int mode);
+
+/* LU::solve_mut (lu.rs:242-260): B (n x nrhs) <- A^{-1} B.  info[0] = 0, or
+ * 1 + the index of an exact zero on U's diagonal (solve_mut returns false and
+ * B holds garbage).  Shape of LUScalar::xgetrs. */
+int nrocm_dgetrs(nrocm_handle handle, int n, int nrhs, const double *lu, int lda,
+                 const int *ipiv, double *b, int ldb, int *info, int mode);
+
+/* ---------------------------------------------------------------- Cholesky (a8-a9)
+ * Cholesky::new (src/linalg/cholesky.rs:196-198,221-272): reads the lower
+ * triangle, writes L there, leaves the strict upper triangle untouched.
+ * info[0] = 0, or j+1 for the first column whose pivot is not > 0 (nalgebra
+ * returns None).  Shape of CholeskyScalar::xpotrf (nalgebra-lapack/src/cholesky.rs:181-197). */
+int nrocm_dpotrf(nrocm_handle handle, int n, double *a, int lda, int *info, int mode);
+/* Cholesky::solve_mut (cholesky.rs:122-129): B <- (L L^T)^{-1} B. */
+int nrocm_dpotrs(nrocm_handle handle, int n, int nrhs, const double *l, int lda,
+                 double *b, int ldb, int mode);
+
+/* ---------------------------------------------------------------- batched, uneven sizes
+ * A batch describes `count` independent systems whose sizes are given on the
+ * host (n_host[j], or m_host[j] for least squares).  Creating it sorts the
+ * members by size, bins them, consults the tuning table for each bin's
+ * schedule and uploads the schedule; calls that use it may be captured in a
+ * hipGraph (done internally on repeated calls with the same pointers).
+ * No reference counterpart: the reference solves a batch as a loop of
+ * Cholesky::new + solve_mut. */
+int nrocm_batch_create(nrocm_handle handle, int count, const int *n_host, int nrhs,
+                       nrocm_batch *batch);
+int nrocm_batch_destroy(nrocm_batch batch);
+/* Use (1) or stop using (0) hipGraph capture + replay for this batch. */
+int nrocm_batch_set_graph(nrocm_batch batch, int enable);
+
+/* For each j: Cholesky::new(A_j) then solve_mut(B_j), in one persistent
+ * launch.  a[j] -> n_j x n_j (lda n_j), b[j] -> n_j x nrhs (ldb n_j); a and b
+ * are device arrays of device pointers.  info[j] as for nrocm_dpotrf; when
+ * info[j] != 0, b[j] is left unchanged. */
+int nrocm_dpotrf_potrs_batched(nrocm_handle handle, nrocm_batch batch, double *const *a,
+                               double *const *b, int *info, int mode);
+
+/* Adjoint of X_j = A_j^{-1} B_j given the factor L_j, the solution X_j and the
+ * upstream gradient dX_j (n_j x nrhs):  Lambda = A^{-1} dX,
+ * dB = Lambda, dA = -1/2 (Lambda X^T + X Lambda^T) (symmetrised, full n x n).
+ * No reference counterpart (nalgebra has no autodiff); built on
+ * Cholesky::solve_mut semantics. */
+int nrocm_dpotrs_backward_batched(nrocm_handle handle, nrocm_batch batch,
+                                  const double *const *l, const double *const *x,
+                                  const double *const *dx, double *const *da,
+                                  double *const *db);
+
+/* ---------------------------------------------------------------- QR (a10-a12)
+ * QR::new (src/linalg/qr.rs:55-76): column i of A holds the unit Householder
+ * axis in rows i..m, diag[i] the signed norm (R_ii = |diag[i]|, qr.rs:81-89),
+ * the strict upper triangle holds R.  Exact nalgebra arithmetic. */
+int nrocm_dgeqrf(nrocm_handle handle, int m, int n, double *a, int lda, double *diag);
+int nrocm_sgeqrf(nrocm_handle handle, int m, int n, float *a, int lda, float *diag);
+/* QR::q_tr_mul (qr.rs:157-171): B (m x nrhs) <- Q^T B. */
+int nrocm_dqr_qtmul(nrocm_handle handle, int m, int n, const double *qr, int lda,
+                    const double *diag, int nrhs, double *b, int ldb);
+/* QR::solve_mut's upper solve (qr.rs:224-256) on the leading n x n block:
+ * B (n x nrhs) <- R^{-1} B.  info[0] = 0 or 1 + index of a zero diag. */
+int nrocm_dqr_rsolve(nrocm_handle handle, int n, const double *qr, int lda,
+                     const double *diag, int nrhs, double *b, int ldb, int *info);
+
+/* Batched tall least squares, min ||A_j x - b_j||, A_j m_j x n (f32, lda m_j),
+ * batch created with the row counts m_j (nrhs = 1).  Blocked compact-WY
+ * Householder QR with FP32 MFMA trailing updates, then x = R^{-1} Q^T b and
+ * `refine` FP64 residual-refinement steps (r = b - A x in FP64, x += R^{-1} Q^T r).
+ * A_j and b_j are overwritten (by the factorization and Q^T b); x is n x count
+ * f64, info[j] = 0 or 1 + index of a zero R diagonal.  Replaces the tall
+ * solve nalgebra-lapack's QrDecomposition::solve provides
+ * (nalgebra-lapack/src/qr/abstraction.rs:48-77, qr_util.rs:167-230). */
+int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *const *a,
+                        float *const *b, double *x, int refine, int *info);
+
+/* ---------------------------------------------------------------- mixed precision (a13)
+ * Solve A x = b (n x n, one right-hand side) by LU with partial pivoting whose
+ * trailing updates run in `precision` (NROCM_PREC_FP32 or NROCM_PREC_BF16 on
+ * MFMA with FP32 accumulation, or NROCM_PREC_FP64) followed by FP64
+ * iterative refinement r = b - A x, A d = r, x += d until the normwise
+ * backward error |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or
+ * maxit steps.  A and b are not modified.  iters_host / berr_host receive
+ * the refinement steps taken and the final backward error (the call
+ * synchronises the stream).  Returns NROCM_ERR_NOT_CONVERGED if tol was
+ * not reached.  No reference counterpart: replaces LU::new + solve_mut
+ * (lu.rs:93,242) for large n. */
+int nrocm_dgesv_ir(nrocm_handle handle, int n, const double *a, int lda, const double *b,
+                   double *x, int precision, double tol, int maxit, int *iters_host,
+                   double *berr_host);
+
+/* ---------------------------------------------------------------- synthetic inputs
+ * Host-side seeded generators (xoshiro256** seeded through SplitMix64) so
+ * that tests and benchmarks on every machine build identical matrices.
+ * No reference counterpart (the reference draws from rand's thread RNG). */
+void nrocm_gen_uniform(uint64_t seed, size_t count, double lo, double hi, double *out_host);
+void nrocm_gen_normal(uint64_t seed, size_t count, double *out_host);
+/* Batched SPD systems: member j (seed + j) draws G (n_j x n_j, N(0,1),
+ * column-major) then B (n_j x nrhs, N(0,1)); A_j = G G^T + n_j I.  Members are
+ * packed back to back: A at offA[j] (n_j^2 doubles), B at offB[j]. */
+void nrocm_gen_spd_batch(uint64_t seed, int count, const int *n_host, int nrhs,
+                         double *a_host, const int64_t *offa, double *b_host,
+                         const int64_t *offb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NALGEBRA_ROCM_H */

@@ -1,0 +1,26 @@
+// Host-side launchers shared between translation units (each kernel template
+// is instantiated in exactly one .hip file).
+#pragma once
+#include "nrocm_internal.hpp"
+
+// gemm.hip
+hipError_t nrocm_gemm_d(hipStream_t st, int m, int n, int k, double alpha, const double *A, long rsa,
+                        long csa, const double *B, long rsb, long csb, double beta, double *C, long rsc,
+                        long csc);
+hipError_t nrocm_gemm_s(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
+                        long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
+                        long csc);
+hipError_t nrocm_gemm_s_bf16(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
+                             long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
+                             long csc);
+
+// lu.hip: trailing-update flavours of the blocked LU.
+enum NrocmTrailing { TRAIL_EXACT = 0, TRAIL_F64 = 1, TRAIL_F32 = 2, TRAIL_BF16 = 3 };
+int nrocm_getrf_d(nrocm_handle h, hipStream_t st, int m, int n, double *A, int lda, int *ipiv, int *info,
+                  NrocmTrailing trail);
+int nrocm_getrf_s(nrocm_handle h, hipStream_t st, int m, int n, float *A, int lda, int *ipiv, int *info,
+                  NrocmTrailing trail);
+
+// chol_batched.hip: n <= 64 single-matrix path.
+int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int ldb, int nrhs, int *info,
+                      bool exact);

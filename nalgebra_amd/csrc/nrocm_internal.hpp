@@ -1,0 +1,69 @@
+// Internal declarations shared by the HIP translation units of libnalgebra_rocm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/nalgebra_rocm.h"
+
+struct nrocm_handle_s {
+    hipStream_t stream = nullptr;   // caller's stream (nullptr = legacy default stream)
+    int device = 0;
+    int num_cus = 0;
+    std::string last_error;
+
+    // Growable scratch on the device, reused across calls (ordered on `stream`).
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    // Small device-side scalars (counters, flags, norms).
+    void *small = nullptr;          // 4 KiB
+};
+
+struct nrocm_batch_s {
+    nrocm_handle handle = nullptr;
+    int count = 0;
+    int nrhs = 0;
+    int max_n = 0;
+    std::vector<int> n_host;        // sizes in caller order
+    std::vector<int> order_host;    // member indices, sorted by decreasing size
+    // Bins: [begin, end) ranges of `order` sharing one kernel instantiation.
+    struct Bin { int nb; int begin; int end; int kernel; };
+    std::vector<Bin> bins;
+    int *order_dev = nullptr;       // device copy of order_host
+    int *n_dev = nullptr;           // device copy of n_host (caller order)
+    // hipGraph replays of captured launch sequences, keyed by their arguments
+    // (a bounded cache, oldest entry evicted first).
+    struct GraphEntry {
+        const void *key[3];
+        int mode;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    bool use_graph = true;
+    std::vector<GraphEntry> graphs;
+    hipStream_t graph_stream = nullptr;
+};
+
+// Records a message on the handle and returns NROCM_ERR_HIP.
+int nrocm_fail_hip(nrocm_handle h, hipError_t e, const char *what);
+int nrocm_fail(nrocm_handle h, int status, const char *msg);
+// Ensures at least `bytes` of workspace; synchronises the stream when it grows.
+int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr);
+
+#define NROCM_CHECK_HIP(h, call)                                    \
+    do {                                                            \
+        hipError_t e__ = (call);                                    \
+        if (e__ != hipSuccess) return nrocm_fail_hip((h), e__, #call); \
+    } while (0)
+
+#define NROCM_CHECK_LAUNCH(h)                                       \
+    do {                                                            \
+        hipError_t e__ = hipGetLastError();                         \
+        if (e__ != hipSuccess) return nrocm_fail_hip((h), e__, "kernel launch"); \
+    } while (0)
+
+static inline unsigned nrocm_cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

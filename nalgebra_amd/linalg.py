@@ -1,0 +1,348 @@
+"""Host-side mirror of nalgebra's dense API for the accelerated path.
+
+Matrices live on the GPU in nalgebra's DMatrix layout (column-major,
+element (i, j) at data[i + j*nrows], src/base/vec_storage.rs:26-32).  Every
+numerical routine is a call into libnalgebra_rocm.so through the C ABI;
+return conventions follow the reference: LU::solve_mut -> bool,
+Matrix::cholesky -> Option (None here), QR::solve_mut -> bool, shape errors
+raise (nalgebra panics).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import EXACT, FAST, check, lib
+
+_handles = {}
+
+
+def handle(device: Optional[int] = None):
+    """The library handle for `device`, bound to torch's current stream."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("nalgebra_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    dev = torch.cuda.current_device() if device is None else device
+    h = _handles.get(dev)
+    if h is None:
+        with torch.cuda.device(dev):
+            hp = C.c_void_p()
+            check("nrocm_create", lib().nrocm_create(C.byref(hp)))
+            h = hp.value
+            _handles[dev] = h
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    check("nrocm_set_stream", lib().nrocm_set_stream(h, stream), h)
+    return h
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+_DT = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32}
+
+
+class DMatrix:
+    """Column-major matrix on the GPU (nalgebra DMatrix storage)."""
+
+    def __init__(self, data: torch.Tensor, nrows: int, ncols: int):
+        if data.dim() != 1 or data.numel() != nrows * ncols:
+            raise ValueError("data must be a flat tensor of nrows*ncols elements")
+        if not data.is_cuda:
+            raise ValueError("DMatrix data must live on the GPU")
+        self.data = data
+        self.nrows = int(nrows)
+        self.ncols = int(ncols)
+
+    # -------------------------------------------------------------- construction
+    @classmethod
+    def from_numpy(cls, a, dtype=None, device=None) -> "DMatrix":
+        a = np.asarray(a, dtype=dtype)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        flat = np.asfortranarray(a).ravel(order="F")
+        t = torch.from_numpy(flat.copy()).to(device or "cuda")
+        return cls(t, a.shape[0], a.shape[1])
+
+    @classmethod
+    def zeros(cls, nrows, ncols, dtype=torch.float64, device="cuda") -> "DMatrix":
+        return cls(torch.zeros(nrows * ncols, dtype=dtype, device=device), nrows, ncols)
+
+    @classmethod
+    def identity(cls, n, dtype=torch.float64, device="cuda") -> "DMatrix":
+        m = cls.zeros(n, n, dtype, device)
+        m.data[:: n + 1] = 1
+        return m
+
+    def clone(self) -> "DMatrix":
+        return DMatrix(self.data.clone(), self.nrows, self.ncols)
+
+    def numpy(self) -> np.ndarray:
+        return self.data.cpu().numpy().reshape(self.ncols, self.nrows).T.copy()
+
+    @property
+    def shape(self):
+        return (self.nrows, self.ncols)
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    def is_square(self) -> bool:
+        return self.nrows == self.ncols
+
+    # -------------------------------------------------------------- BLAS
+    def gemm(self, alpha, a: "DMatrix", b: "DMatrix", beta) -> None:
+        """self = alpha * a * b + beta * self (src/base/blas.rs:729)."""
+        gemm(self, alpha, a, b, beta)
+
+    def __matmul__(self, other: "DMatrix") -> "DMatrix":
+        out = DMatrix.zeros(self.nrows, other.ncols, self.dtype, self.data.device)
+        gemm(out, 1.0, self, other, 0.0)
+        return out
+
+    # -------------------------------------------------------------- decompositions
+    def lu(self, mode: int = EXACT) -> "LU":
+        """LU with partial pivoting of a copy of self (decomposition.rs:48)."""
+        return LU(self.clone(), mode)
+
+    def cholesky(self, mode: int = EXACT) -> Optional["Cholesky"]:
+        """Cholesky of a copy of self; None when not positive definite
+        (decomposition.rs:257, cholesky.rs:196)."""
+        return Cholesky.new(self.clone(), mode)
+
+    def qr(self) -> "QR":
+        """Householder QR of a copy of self (decomposition.rs:57)."""
+        return QR(self.clone())
+
+
+def gemm(c: DMatrix, alpha, a: DMatrix, b: DMatrix, beta) -> None:
+    if a.ncols != b.nrows:
+        raise ValueError("gemm: dimensions mismatch for multiplication.")
+    if (c.nrows, c.ncols) != (a.nrows, b.ncols):
+        raise ValueError("gemm: dimensions mismatch for addition.")
+    if not (a.dtype == b.dtype == c.dtype):
+        raise TypeError("gemm: mixed dtypes")
+    h = handle()
+    fn = lib().nrocm_dgemm if c.dtype == torch.float64 else lib().nrocm_sgemm
+    st = fn(h, a.nrows, a.ncols, b.ncols, alpha, _ptr(a.data), 1, a.nrows, _ptr(b.data), 1,
+            b.nrows, beta, _ptr(c.data), 1, c.nrows)
+    check(fn.__name__, st, h)
+
+
+class PermutationSequence:
+    """The row transpositions of an LU, as nalgebra records them
+    (src/linalg/permutation_sequence.rs:28-34,84-93)."""
+
+    def __init__(self, ipiv: np.ndarray):
+        self.ipiv = [(i, int(p)) for i, p in enumerate(ipiv) if int(p) != i]
+
+    def __len__(self):
+        return len(self.ipiv)
+
+    def determinant(self) -> float:
+        return -1.0 if len(self.ipiv) % 2 else 1.0
+
+    def permute_rows(self, m: np.ndarray) -> None:
+        for i, p in self.ipiv:
+            m[[i, p]] = m[[p, i]]
+
+    def inv_permute_rows(self, m: np.ndarray) -> None:
+        for i, p in reversed(self.ipiv):
+            m[[i, p]] = m[[p, i]]
+
+
+class LU:
+    """PA = LU with partial pivoting (src/linalg/lu.rs:32-38,93-122)."""
+
+    def __init__(self, m: DMatrix, mode: int = EXACT):
+        if m.dtype != torch.float64:
+            raise TypeError("LU: float64 only")
+        self.lu = m
+        self.mode = mode
+        k = min(m.nrows, m.ncols)
+        dev = m.data.device
+        self.ipiv = torch.zeros(max(k, 1), dtype=torch.int32, device=dev)
+        self._info = torch.zeros(1, dtype=torch.int32, device=dev)
+        h = handle()
+        st = lib().nrocm_dgetrf(h, m.nrows, m.ncols, _ptr(m.data), max(m.nrows, 1), _ptr(self.ipiv),
+                                _ptr(self._info), mode)
+        check("nrocm_dgetrf", st, h)
+
+    @property
+    def info(self) -> int:
+        return int(self._info.item())
+
+    @property
+    def p(self) -> PermutationSequence:
+        k = min(self.lu.nrows, self.lu.ncols)
+        return PermutationSequence(self.ipiv[:k].cpu().numpy())
+
+    def solve_mut(self, b: DMatrix) -> bool:
+        """LU::solve_mut (lu.rs:242-260): False when U is singular."""
+        if self.lu.nrows != b.nrows:
+            raise ValueError("LU solve matrix dimension mismatch.")
+        if not self.lu.is_square():
+            raise ValueError("LU solve: unable to solve a non-square system.")
+        info = torch.zeros(1, dtype=torch.int32, device=b.data.device)
+        h = handle()
+        st = lib().nrocm_dgetrs(h, self.lu.nrows, b.ncols, _ptr(self.lu.data), self.lu.nrows,
+                                _ptr(self.ipiv), _ptr(b.data), b.nrows, _ptr(info), self.mode)
+        check("nrocm_dgetrs", st, h)
+        return int(info.item()) == 0
+
+    def solve(self, b: DMatrix) -> Optional[DMatrix]:
+        x = b.clone()
+        return x if self.solve_mut(x) else None
+
+    def l(self) -> np.ndarray:
+        a = self.lu.numpy()
+        k = min(a.shape)
+        L = np.tril(a[:, :k], -1)
+        L[np.arange(k), np.arange(k)] = 1.0
+        return L
+
+    def u(self) -> np.ndarray:
+        a = self.lu.numpy()
+        k = min(a.shape)
+        return np.triu(a[:k, :])
+
+    def determinant(self) -> float:
+        """LU::determinant (lu.rs:301-314)."""
+        if not self.lu.is_square():
+            raise ValueError("LU determinant: non-square matrix.")
+        d = torch.diagonal(self.lu.data.view(self.lu.ncols, self.lu.nrows)).cpu().numpy()
+        res = 1.0
+        for v in d:
+            res = res * float(v)
+        return res * self.p.determinant()
+
+    def try_inverse(self) -> Optional[DMatrix]:
+        """LU::try_inverse (lu.rs:266-297)."""
+        out = DMatrix.identity(self.lu.nrows, device=self.lu.data.device)
+        return out if self.solve_mut(out) else None
+
+
+class Cholesky:
+    """A = L L^T of an SPD matrix (src/linalg/cholesky.rs:28-33)."""
+
+    def __init__(self, chol: DMatrix, mode: int):
+        self.chol = chol
+        self.mode = mode
+
+    @classmethod
+    def new(cls, m: DMatrix, mode: int = EXACT) -> Optional["Cholesky"]:
+        if not m.is_square():
+            raise ValueError("The input matrix must be square.")
+        if m.dtype != torch.float64:
+            raise TypeError("Cholesky: float64 only")
+        info = torch.zeros(1, dtype=torch.int32, device=m.data.device)
+        h = handle()
+        st = lib().nrocm_dpotrf(h, m.nrows, _ptr(m.data), max(m.nrows, 1), _ptr(info), mode)
+        check("nrocm_dpotrf", st, h)
+        return None if int(info.item()) != 0 else cls(m, mode)
+
+    def l_dirty(self) -> DMatrix:
+        return self.chol
+
+    def l(self) -> np.ndarray:
+        return np.tril(self.chol.numpy())
+
+    def solve_mut(self, b: DMatrix) -> None:
+        """Cholesky::solve_mut (cholesky.rs:122-129)."""
+        if b.nrows != self.chol.nrows:
+            raise ValueError("Cholesky solve: dimension mismatch")
+        h = handle()
+        st = lib().nrocm_dpotrs(h, self.chol.nrows, b.ncols, _ptr(self.chol.data), self.chol.nrows,
+                                _ptr(b.data), b.nrows, self.mode)
+        check("nrocm_dpotrs", st, h)
+
+    def solve(self, b: DMatrix) -> DMatrix:
+        x = b.clone()
+        self.solve_mut(x)
+        return x
+
+    def backward(self, x: DMatrix, dx: DMatrix):
+        """Adjoint of x = A^{-1} b: returns (dA symmetrised, db) for upstream dx."""
+        from .batched import CholeskyBatch
+        n = self.chol.nrows
+        bt = CholeskyBatch([n], nrhs=x.ncols, device=self.chol.data.device)
+        dA = DMatrix.zeros(n, n, device=self.chol.data.device)
+        db = DMatrix.zeros(n, x.ncols, device=self.chol.data.device)
+        bt.backward_ptrs([self.chol.data], [x.data], [dx.data], [dA.data], [db.data])
+        return dA, db
+
+
+class QR:
+    """Householder QR in nalgebra's representation (src/linalg/qr.rs:34-40):
+    unit axes below (and on) the diagonal, signed norms in diag."""
+
+    def __init__(self, m: DMatrix):
+        self.qr = m
+        k = min(m.nrows, m.ncols)
+        self.diag = torch.zeros(max(k, 1), dtype=m.dtype, device=m.data.device)
+        h = handle()
+        fn = lib().nrocm_dgeqrf if m.dtype == torch.float64 else lib().nrocm_sgeqrf
+        st = fn(h, m.nrows, m.ncols, _ptr(m.data), max(m.nrows, 1), _ptr(self.diag))
+        check(fn.__name__, st, h)
+
+    def q_tr_mul(self, b: DMatrix) -> None:
+        """QR::q_tr_mul (qr.rs:157-171), in place."""
+        if self.qr.dtype != torch.float64:
+            raise TypeError("q_tr_mul: float64 only")
+        h = handle()
+        st = lib().nrocm_dqr_qtmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), self.qr.nrows,
+                                   _ptr(self.diag), b.ncols, _ptr(b.data), b.nrows)
+        check("nrocm_dqr_qtmul", st, h)
+
+    def solve_mut(self, b: DMatrix) -> bool:
+        """QR::solve_mut (qr.rs:204-221): square systems only."""
+        if self.qr.nrows != b.nrows:
+            raise ValueError("QR solve matrix dimension mismatch.")
+        if not self.qr.is_square():
+            raise ValueError("QR solve: unable to solve a non-square system.")
+        self.q_tr_mul(b)
+        return self._rsolve(b)
+
+    def _rsolve(self, b: DMatrix) -> bool:
+        info = torch.zeros(1, dtype=torch.int32, device=b.data.device)
+        h = handle()
+        st = lib().nrocm_dqr_rsolve(h, self.qr.ncols, _ptr(self.qr.data), self.qr.nrows,
+                                    _ptr(self.diag), b.ncols, _ptr(b.data), b.nrows, _ptr(info))
+        check("nrocm_dqr_rsolve", st, h)
+        return int(info.item()) == 0
+
+    def solve_least_squares(self, b: DMatrix) -> Optional[np.ndarray]:
+        """Tall least squares: Q^T b then the upper solve on rows 0..n."""
+        y = b.clone()
+        self.q_tr_mul(y)
+        n = self.qr.ncols
+        top = DMatrix(y.data.view(y.ncols, y.nrows)[:, :n].contiguous().view(-1), n, y.ncols)
+        return top if self._rsolve(top) else None
+
+    def r(self) -> np.ndarray:
+        a = self.qr.numpy()
+        k = min(a.shape)
+        R = np.triu(a[:k, :])
+        R[np.arange(k), np.arange(k)] = np.abs(self.diag[:k].cpu().numpy())
+        return R
+
+
+def solve_mixed_ir(a: DMatrix, b: DMatrix, precision: int = _lib.PREC_FP32, tol: float = 1e-14,
+                   maxit: int = 30):
+    """Mixed-precision LU + FP64 iterative refinement.  Returns (x, iters, berr, converged)."""
+    if not a.is_square() or b.nrows != a.nrows or b.ncols != 1:
+        raise ValueError("solve_mixed_ir: A must be n x n and b n x 1")
+    x = DMatrix.zeros(a.nrows, 1, device=a.data.device)
+    iters = C.c_int(0)
+    berr = C.c_double(0.0)
+    h = handle()
+    st = lib().nrocm_dgesv_ir(h, a.nrows, _ptr(a.data), a.nrows, _ptr(b.data), _ptr(x.data), precision,
+                              tol, maxit, C.byref(iters), C.byref(berr))
+    if st not in (_lib.OK, _lib.ERR_NOT_CONVERGED):
+        check("nrocm_dgesv_ir", st, h)
+    return x, iters.value, berr.value, st == _lib.OK

@@ -1,0 +1,168 @@
+"""GPU parity for Cholesky + solve (single and batched uneven) and the adjoint.
+
+Bar: NROCM_EXACT is bit-identical to the oracle (nalgebra's operation order);
+NROCM_FAST is within 1e-12 relative (Frobenius) of it.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def _rel(a, b):
+    d = np.linalg.norm(np.asarray(a) - np.asarray(b))
+    return d / max(np.linalg.norm(b), 1e-300)
+
+
+NS_ALL = list(range(1, 65)) + [64] * 9 + [33] * 5 + [4] * 11 + list(range(64, 0, -7))
+
+
+def _run_batch(nl, ns, nrhs, seed, mode, graph=True):
+    A, offA, B, offB = gen.spd_batch(seed, ns, nrhs)
+    bt = nl.CholeskyBatch(ns, nrhs)
+    bt.set_graph(graph)
+    Ad = torch.from_numpy(A).cuda()
+    Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, mode)
+    torch.cuda.synchronize()
+    return A, offA, B, offB, Ad.cpu().numpy(), Bd.cpu().numpy(), info.cpu().numpy()
+
+
+@pytest.mark.parametrize("nrhs", [4, 1, 6])
+def test_batched_exact_is_bitwise(nl, orc, nrhs):
+    ns = NS_ALL
+    print(f"batched exact: {len(ns)} members, nrhs={nrhs}", flush=True)
+    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 11, nl.EXACT)
+    Ar, Br = A.copy(), B.copy()
+    info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
+    assert np.array_equal(info, info_r) and not info.any()
+    assert np.array_equal(_bits(Ag), _bits(Ar)), "factor bits differ"
+    assert np.array_equal(_bits(Bg), _bits(Br)), "solution bits differ"
+
+
+def test_batched_fast_within_1e12(nl, orc):
+    ns = NS_ALL
+    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST)
+    Ar, Br = A.copy(), B.copy()
+    orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
+    assert not info.any()
+    for j, n in enumerate(ns):
+        xa = Bg[offB[j]:offB[j] + 4 * n]; xr = Br[offB[j]:offB[j] + 4 * n]
+        assert _rel(xa, xr) <= 1e-12, (j, n, _rel(xa, xr))
+
+
+def test_batched_not_positive_definite(nl, orc):
+    ns = [5, 17, 40, 64, 3]
+    A, offA, B, offB = gen.spd_batch(13, ns, 4)
+    # member 2: make column 7's pivot negative after elimination; member 4: NaN diagonal
+    M = A[offA[2]:offA[2] + 1600].reshape(40, 40, order="F")
+    M[7, 7] = -100.0
+    A[offA[4]] = np.nan
+    bt = nl.CholeskyBatch(ns, 4)
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
+    Ar, Br = A.copy(), B.copy()
+    info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
+    assert list(info) == list(info_r) == [0, 0, 8, 0, 1]
+    Bg = Bd.cpu().numpy()
+    # a failed member's right-hand side is left unchanged; others are solved exactly
+    assert np.array_equal(Bg[offB[2]:offB[2] + 160], B[offB[2]:offB[2] + 160])
+    for j in (0, 1, 3):
+        sl = slice(offB[j], offB[j] + 4 * ns[j])
+        assert np.array_equal(_bits(Bg[sl]), _bits(Br[sl]))
+
+
+def test_batched_graph_replays_match_eager(nl):
+    ns = [64, 50, 31, 17, 8, 2] * 20
+    A, offA, B, offB = gen.spd_batch(14, ns, 4)
+    bt = nl.CholeskyBatch(ns, 4)
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    ptrs = bt.pointers(Ad, Bd)
+    bt.set_graph(False)
+    bt.solve(Ad, Bd, nl.EXACT, ptrs=ptrs)
+    torch.cuda.synchronize()
+    eager = Bd.cpu().numpy().copy()
+    bt.set_graph(True)
+    for rep in range(4):
+        Ad.copy_(torch.from_numpy(A)); Bd.copy_(torch.from_numpy(B))
+        info = bt.solve(Ad, Bd, nl.EXACT, ptrs=ptrs)
+        torch.cuda.synchronize()
+        assert not info.cpu().numpy().any()
+        assert np.array_equal(_bits(Bd.cpu().numpy()), _bits(eager)), f"replay {rep} differs"
+        _ = torch.ones(10, device="cuda").sum().item()   # unrelated eager work between replays
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 130, 257])
+def test_single_cholesky_exact(nl, orc, n):
+    G = gen.normal(20 + n, (n, n))
+    A = G @ G.T + n * np.eye(n)
+    A[0, n - 1] = 12345.0 if n > 1 else A[0, 0]             # upper triangle must not be read
+    b = gen.normal(30 + n, (n, 3))
+    ch = nl.DMatrix.from_numpy(A).cholesky(nl.EXACT)
+    assert ch is not None
+    Lr, info = orc.potrf(A)
+    assert info == 0
+    Lg = ch.l_dirty().numpy()
+    assert np.array_equal(_bits(Lg), _bits(Lr))
+    x = ch.solve(nl.DMatrix.from_numpy(b)).numpy()
+    xr = orc.potrs(Lr, b)
+    assert np.array_equal(_bits(x), _bits(xr))
+
+
+def test_single_cholesky_none(nl):
+    A = np.array([[1.0, 2.0], [2.0, 1.0]])
+    assert nl.DMatrix.from_numpy(A).cholesky() is None
+    A = np.eye(100); A[70, 70] = -1.0
+    assert nl.DMatrix.from_numpy(A).cholesky() is None
+
+
+# ------------------------------------------------------------------ adjoint
+
+def test_adjoint_exact_matches_oracle(nl, orc):
+    ns = [32] * 64 + [5, 17, 64, 1]
+    A, offA, B, offB = gen.spd_batch(40, ns, 1)
+    bt = nl.CholeskyBatch(ns, 1)
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, nl.EXACT)
+    dX = 2.0 * Bd                                      # loss = sum |x|^2
+    dA, dB = bt.backward(Ad, Bd, dX)
+    torch.cuda.synchronize()
+    assert not info.cpu().numpy().any()
+    Lg = Ad.cpu().numpy(); Xg = Bd.cpu().numpy(); dAg = dA.cpu().numpy(); dBg = dB.cpu().numpy()
+    for j, n in enumerate(ns):
+        L = Lg[offA[j]:offA[j] + n * n].reshape(n, n, order="F")
+        x = Xg[offB[j]:offB[j] + n]
+        dAr, dBr = orc.potrs_adjoint(L, x, 2 * x)
+        assert np.array_equal(dBg[offB[j]:offB[j] + n], dBr)
+        assert np.array_equal(dAg[offA[j]:offA[j] + n * n].reshape(n, n, order="F"), dAr)
+
+
+def test_adjoint_central_differences(nl):
+    n = 32
+    G = gen.normal(41, (n, n)); A = G @ G.T + n * np.eye(n); b = gen.normal(42, (n,))
+
+    def loss(Am, bv):
+        ch = nl.DMatrix.from_numpy(Am).cholesky(nl.EXACT)
+        x = ch.solve(nl.DMatrix.from_numpy(bv.reshape(-1, 1))).numpy()[:, 0]
+        return float(np.sum(x * x)), ch, x
+
+    _, ch, x = loss(A, b)
+    dA, db = ch.backward(nl.DMatrix.from_numpy(x.reshape(-1, 1)), nl.DMatrix.from_numpy(2 * x.reshape(-1, 1)))
+    dA = dA.numpy(); db = db.numpy()[:, 0]
+    h = 1e-6
+    for (i, j) in [(0, 0), (5, 3), (31, 31), (20, 7), (31, 0)]:
+        Ap = A.copy(); Am = A.copy(); Ap[i, j] += h; Am[i, j] -= h
+        fd = (loss(Ap, b)[0] - loss(Am, b)[0]) / (2 * h)
+        an = dA[i, j] * (1 if i == j else 2)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (i, j, fd, an)
+    for i in (0, 13, 31):
+        bp = b.copy(); bm = b.copy(); bp[i] += h; bm[i] -= h
+        fd = (loss(A, bp)[0] - loss(A, bm)[0]) / (2 * h)
+        assert abs(fd - db[i]) <= 1e-6 * max(1.0, abs(db[i])), (i, fd, db[i])
