@@ -44,6 +44,7 @@ extern "C" {
 #define NROCM_PREC_FP64 0
 #define NROCM_PREC_FP32 1
 #define NROCM_PREC_BF16 2
+#define NROCM_PREC_BF16X3 3  /* BF16 MFMA on three-way split operands (~FP32 accuracy) */
 
 typedef struct nrocm_handle_s *nrocm_handle;
 typedef struct nrocm_batch_s *nrocm_batch;
@@ -159,14 +160,15 @@ int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *co
 
 /* ---------------------------------------------------------------- mixed precision (a13)
  * Solve A x = b (n x n, one right-hand side) by LU with partial pivoting whose
- * trailing updates run in `precision` (NROCM_PREC_FP32 or NROCM_PREC_BF16 on
- * MFMA with FP32 accumulation, or NROCM_PREC_FP64) followed by FP64
- * iterative refinement r = b - A x, A d = r, x += d until the normwise
- * backward error |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or
- * maxit steps.  A and b are not modified.  iters_host / berr_host receive
- * the refinement steps taken and the final backward error (the call
- * synchronises the stream).  Returns NROCM_ERR_NOT_CONVERGED if tol was
- * not reached.  No reference counterpart: replaces LU::new + solve_mut
+ * trailing updates run in `precision`: NROCM_PREC_FP32, NROCM_PREC_BF16 or
+ * NROCM_PREC_BF16X3 on MFMA with FP32 accumulation (FP32 working copy, FP32
+ * panels, pivot magnitudes compared in FP64), or NROCM_PREC_FP64 (FP64 MFMA),
+ * followed by FP64 iterative refinement r = b - A x, A d = r, x += d until the
+ * normwise backward error |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol
+ * or maxit solves.  A and b are not modified.  iters_host / berr_host receive
+ * the solves taken and the final backward error (the call synchronises the
+ * stream once per refinement step).  Returns NROCM_ERR_NOT_CONVERGED if tol
+ * was not reached.  No reference counterpart: replaces LU::new + solve_mut
  * (lu.rs:93,242) for large n. */
 int nrocm_dgesv_ir(nrocm_handle handle, int n, const double *a, int lda, const double *b,
                    double *x, int precision, double tol, int maxit, int *iters_host,

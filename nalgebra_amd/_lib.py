@@ -23,6 +23,7 @@ FAST = 1
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
+PREC_BF16X3 = 3
 
 _vp = C.c_void_p
 _i = C.c_int

@@ -27,6 +27,13 @@ hipError_t nrocm_gemm_s_bf16(hipStream_t st, int m, int n, int k, float alpha, c
     return gemm<float, MODE_BF16>(st, m, n, k, alpha, A, rsa, csa, B, rsb, csb, beta, C, rsc, csc);
 }
 
+hipError_t nrocm_gemm_s_bf16x3(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
+                               long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
+                               long csc)
+{
+    return gemm<float, MODE_BF16X3>(st, m, n, k, alpha, A, rsa, csa, B, rsb, csb, beta, C, rsc, csc);
+}
+
 namespace {
 
 template <typename T, typename F>

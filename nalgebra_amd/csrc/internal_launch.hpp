@@ -14,8 +14,12 @@ hipError_t nrocm_gemm_s_bf16(hipStream_t st, int m, int n, int k, float alpha, c
                              long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
                              long csc);
 
+hipError_t nrocm_gemm_s_bf16x3(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
+                               long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
+                               long csc);
+
 // lu.hip: trailing-update flavours of the blocked LU.
-enum NrocmTrailing { TRAIL_EXACT = 0, TRAIL_F64 = 1, TRAIL_F32 = 2, TRAIL_BF16 = 3 };
+enum NrocmTrailing { TRAIL_EXACT = 0, TRAIL_F64 = 1, TRAIL_F32 = 2, TRAIL_BF16 = 3, TRAIL_BF16X3 = 4 };
 int nrocm_getrf_d(nrocm_handle h, hipStream_t st, int m, int n, double *A, int lda, int *ipiv, int *info,
                   NrocmTrailing trail);
 int nrocm_getrf_s(nrocm_handle h, hipStream_t st, int m, int n, float *A, int lda, int *ipiv, int *info,

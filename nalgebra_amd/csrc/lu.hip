@@ -112,6 +112,9 @@ hipError_t trailing(hipStream_t st, NrocmTrailing trail, T *A, int lda, int r0, 
         return nrocm_gemm_d(st, mm, nn, K, -1.0, (const double *)Lp, 1, lda, (const double *)Up, 1, lda, 1.0,
                             (double *)C, 1, lda);
     } else {
+        if (trail == TRAIL_BF16X3)
+            return nrocm_gemm_s_bf16x3(st, mm, nn, K, -1.0f, (const float *)Lp, 1, lda, (const float *)Up, 1, lda,
+                                       1.0f, (float *)C, 1, lda);
         if (trail == TRAIL_BF16)
             return nrocm_gemm_s_bf16(st, mm, nn, K, -1.0f, (const float *)Lp, 1, lda, (const float *)Up, 1, lda,
                                      1.0f, (float *)C, 1, lda);

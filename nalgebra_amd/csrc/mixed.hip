@@ -1,12 +1,371 @@
-// Mixed-precision LU + FP64 iterative refinement.
+// Mixed-precision LU + FP64 iterative refinement (nrocm_dgesv_ir).
+//
+//   A32 = fl32(A)                      working copy (FP64 kept for residuals)
+//   P A32 = L U                        blocked LU, trailing updates on FP32 or
+//                                      BF16 MFMA (FP32 accumulate); FP64 mode
+//                                      factors a copy of A with FP64 MFMA
+//   x = 0, r = b
+//   repeat:  d = U^{-1} L^{-1} P r     (working precision, single-launch
+//                                       look-back triangular solves)
+//            x += d                    (FP64)
+//            r = b - A x               (FP64, deterministic two-pass GEMV)
+//            berr = |r|_inf / (|A|_inf |x|_inf + |b|_inf)
+//   until berr <= tol or maxit.
 #include "internal_launch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int BS = 64;           // triangular-solve block
+constexpr int GEMV_CS = 8;       // column splits of the residual GEMV
+constexpr long SPIN_LIMIT = 1L << 26;
+
+__global__ void cvt_d2s_kernel(const double *__restrict__ a, int lda, float *__restrict__ o, int ldo, int n)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)n * n;
+    if (idx >= tot) return;
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    o[r + (size_t)c * ldo] = (float)a[r + (size_t)c * lda];
+}
+
+__global__ void copy_d_kernel(const double *__restrict__ a, int lda, double *__restrict__ o, int ldo, int n)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)n * n;
+    if (idx >= tot) return;
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    o[r + (size_t)c * ldo] = a[r + (size_t)c * lda];
+}
+
+// Final row order of P: perm[i] = original row that ends at position i.
+__global__ void perm_kernel(const int *ipiv, int n, int *perm)
+{
+    extern __shared__ int p[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i) {
+            const int q = ipiv[i];
+            if (q != i) {
+                const int t = p[i];
+                p[i] = p[q];
+                p[q] = t;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = p[i];
+}
+
+template <typename T>
+__global__ void gather_kernel(const double *__restrict__ r, const int *__restrict__ perm, T *__restrict__ o, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (T)r[perm[i]];
+}
+
+__global__ void zero_int_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Single-launch blocked triangular solve of L y = v (unit lower, LOWER) or
+// U y = v (non-unit upper).  Work items are 64-row blocks, taken in solve order
+// through an atomic ticket, so every block a workgroup waits for is owned by a
+// workgroup that is already running.  Waits are bounded: on timeout the
+// error word is set and the kernel still exits.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict__ M, int lda, int n, T *v, int *ticket,
+                                                            int *flags, int *err)
+{
+    __shared__ T xs[BS];
+    __shared__ T part[4][BS];
+    __shared__ T D[BS][BS + 1];
+    __shared__ int my;
+    const int tid = threadIdx.x;
+    const int nblk = (n + BS - 1) / BS;
+    if (tid == 0) my = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int k = my;                               // k-th block in solve order
+    const int R = LOWER ? k : nblk - 1 - k;         // row block
+    const int r0 = R * BS;
+    const int rows = min(BS, n - r0);
+    const int rr = tid & 63, sl = tid >> 6;         // row within block, column slice
+    T acc = 0;
+    for (int j = 0; j < k; ++j) {
+        const int J = LOWER ? j : nblk - 1 - j;
+        const int c0 = J * BS;
+        const int cols = min(BS, n - c0);
+        if (tid == 0) {
+            long spins = 0;
+            while (__hip_atomic_load(&flags[J], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                if (++spins > SPIN_LIMIT) {
+                    atomicExch(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (tid < cols) xs[tid] = __hip_atomic_load(&v[c0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (rr < rows) {
+            const T *col = M + (r0 + rr) + (size_t)c0 * lda;
+            for (int c = sl; c < cols; c += 4) acc = fma(-col[(size_t)c * lda], xs[c], acc);
+        }
+        __syncthreads();
+    }
+    part[sl][rr] = acc;
+    // diagonal block
+    for (int e = tid; e < BS * BS; e += 256) {
+        const int a = e % BS, b = e / BS;
+        D[a][b] = (a < rows && b < rows) ? M[(r0 + a) + (size_t)(r0 + b) * lda] : T(0);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        T y = 0;
+        if (tid < rows) y = v[r0 + tid] + ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
+        if (LOWER) {
+            for (int i = 0; i < rows; ++i) {
+                const T yi = __shfl(y, i);
+                if (tid > i && tid < rows) y = fma(-D[tid][i], yi, y);
+            }
+        } else {
+            for (int i = rows - 1; i >= 0; --i) {
+                if (tid == i) y = y / D[i][i];
+                const T yi = __shfl(y, i);
+                if (tid < i) y = fma(-D[tid][i], yi, y);
+            }
+        }
+        if (tid < rows) __hip_atomic_store(&v[r0 + tid], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(&flags[R], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// partial[cs][i] = sum over the cs-th column slice of A[i, c] * x[c]  (or |A| * 1)
+template <bool ABS>
+__global__ __launch_bounds__(256) void gemv_part_kernel(const double *__restrict__ A, int lda, int n,
+                                                        const double *__restrict__ x, double *__restrict__ part)
+{
+    __shared__ double red[4][64];
+    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int sl = threadIdx.x >> 6;
+    const int cs = blockIdx.y;
+    const int cper = (n + GEMV_CS - 1) / GEMV_CS;
+    const int c0 = cs * cper, c1 = min(n, c0 + cper);
+    double acc = 0.0;
+    if (r < n) {
+        for (int c = c0 + sl; c < c1; c += 4) {
+            const double a = A[r + (size_t)c * lda];
+            acc = ABS ? acc + fabs(a) : fma(a, x[c], acc);
+        }
+    }
+    red[sl][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (sl == 0 && r < n) {
+        const int t = threadIdx.x;
+        part[(size_t)cs * n + r] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    }
+}
+
+// r = b - sum_cs part[cs]  (fixed order), and max |r_i| via ordered-bits atomicMax.
+__global__ void residual_kernel(const double *__restrict__ b, const double *__restrict__ part, int n,
+                                double *__restrict__ r, unsigned long long *rmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int cs = 0; cs < GEMV_CS; ++cs) s += part[(size_t)cs * n + i];
+    const double ri = b[i] - s;
+    r[i] = ri;
+    atomicMax(rmax, (unsigned long long)__double_as_longlong(fabs(ri)));
+}
+
+__global__ void rowsum_max_kernel(const double *__restrict__ part, int n, unsigned long long *amax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int cs = 0; cs < GEMV_CS; ++cs) s += part[(size_t)cs * n + i];
+    atomicMax(amax, (unsigned long long)__double_as_longlong(s));
+}
+
+template <typename T>
+__global__ void update_x_kernel(double *__restrict__ x, const T *__restrict__ d, int n, unsigned long long *xmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i] + (double)d[i];
+    x[i] = xi;
+    atomicMax(xmax, (unsigned long long)__double_as_longlong(fabs(xi)));
+}
+
+__global__ void absmax_kernel(const double *__restrict__ v, int n, unsigned long long *m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMax(m, (unsigned long long)__double_as_longlong(fabs(v[i])));
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void zero_d_kernel(double *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+#define IR_TRY(x)                                                       \
+    do {                                                                \
+        hipError_t e__ = (x);                                           \
+        if (e__ != hipSuccess) return nrocm_fail_hip(h, e__, #x);       \
+    } while (0)
+
+template <typename T>
+int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, T *v, int *sync, int *err)
+{
+    const int nblk = (n + BS - 1) / BS;
+    int *ticket = sync, *flags = sync + 1;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL((trsv_lookback_kernel<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, n, v, ticket, flags,
+                       err);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL((trsv_lookback_kernel<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, n, v, ticket, flags,
+                       err);
+    IR_TRY(hipGetLastError());
+    return NROCM_OK;
+}
+
+template <typename T>
+int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, NrocmTrailing trail,
+            double tol, int maxit, int *iters, double *berr_out)
+{
+    hipStream_t st = h->stream;
+    const int nblk = (n + BS - 1) / BS;
+    // workspace layout
+    const size_t szW = (size_t)n * n * sizeof(T);
+    const size_t szVec = (size_t)n * sizeof(double);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t oW = take(szW), oIpiv = take(n * sizeof(int)), oPerm = take(n * sizeof(int)),
+                 oR = take(szVec), oD = take((size_t)n * sizeof(T)), oPart = take(GEMV_CS * szVec),
+                 oSync = take((nblk + 2) * sizeof(int)), oRed = take(8 * sizeof(unsigned long long)),
+                 oInfo = take(2 * sizeof(int));
+    void *ws = nullptr;
+    int s = nrocm_workspace(h, off, &ws);
+    if (s != NROCM_OK) return s;
+    char *base = (char *)ws;
+    T *W = (T *)(base + oW);
+    int *ipiv = (int *)(base + oIpiv), *perm = (int *)(base + oPerm);
+    double *r = (double *)(base + oR), *part = (double *)(base + oPart);
+    T *d = (T *)(base + oD);
+    int *sync = (int *)(base + oSync);
+    unsigned long long *red = (unsigned long long *)(base + oRed);  // [0]=|A|, [1]=|b|, [2]=|r|, [3]=|x|
+    int *info = (int *)(base + oInfo), *err = info + 1;
+
+    const size_t tot = (size_t)n * n;
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL(cvt_d2s_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, W, n, n);
+    else
+        hipLaunchKernelGGL(copy_d_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, (double *)W, n, n);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, info, 2);
+    IR_TRY(hipGetLastError());
+    int fs;
+    if constexpr (sizeof(T) == 4)
+        fs = nrocm_getrf_s(h, st, n, n, (float *)W, n, ipiv, info, trail);
+    else
+        fs = nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
+    if (fs != NROCM_OK) return fs;
+    hipLaunchKernelGGL(perm_kernel, dim3(1), dim3(1024), n * sizeof(int), st, ipiv, n, perm);
+    IR_TRY(hipGetLastError());
+    // norms of A (row sums) and b
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red, 8);
+    hipLaunchKernelGGL((gemv_part_kernel<true>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda, n,
+                       nullptr, part);
+    hipLaunchKernelGGL(rowsum_max_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, part, n, red + 0);
+    hipLaunchKernelGGL(absmax_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, b, n, red + 1);
+    hipLaunchKernelGGL(zero_d_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, n);
+    IR_TRY(hipGetLastError());
+    IR_TRY(hipMemcpyAsync(r, b, szVec, hipMemcpyDeviceToDevice, st));
+
+    double berr = INFINITY;
+    int it = 0;
+    unsigned long long hred[4];
+    int hinfo[2];
+    for (it = 1; it <= maxit; ++it) {
+        hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
+        IR_TRY(hipGetLastError());
+        s = trsv_pair<T>(h, st, W, n, n, d, sync, err);
+        if (s != NROCM_OK) return s;
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
+        hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
+        hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda, n, x,
+                           part);
+        hipLaunchKernelGGL(residual_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, b, part, n, r, red + 2);
+        IR_TRY(hipGetLastError());
+        IR_TRY(hipMemcpyAsync(hred, red, sizeof(hred), hipMemcpyDeviceToHost, st));
+        IR_TRY(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+        IR_TRY(hipStreamSynchronize(st));
+        if (hinfo[1]) return nrocm_fail(h, NROCM_ERR_HIP, "triangular solve wait timed out");
+        double v[4];
+        for (int q = 0; q < 4; ++q) {
+            long long bits = (long long)hred[q];
+            std::memcpy(&v[q], &bits, sizeof(double));
+        }
+        berr = v[2] / (v[0] * v[3] + v[1]);
+        if (!(berr > tol)) break;  // also stops on NaN
+    }
+    if (it > maxit) it = maxit;
+    if (iters) *iters = it;
+    if (berr_out) *berr_out = berr;
+    if (hinfo[0] != 0) return nrocm_fail(h, NROCM_ERR_NOT_CONVERGED, "singular factorization");
+    return berr <= tol ? NROCM_OK : NROCM_ERR_NOT_CONVERGED;
+}
+
+}  // namespace
 
 extern "C" {
 
-int nrocm_dgesv_ir(nrocm_handle h, int, const double *, int, const double *, double *, int, double, int, int *,
-                   double *)
+int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double *b, double *x, int precision,
+                   double tol, int maxit, int *iters_host, double *berr_host)
 {
-    return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "dgesv_ir: not built yet");
+    if (!h) return -1;
+    if (n < 0) return -2;
+    if (!a && n > 0) return -3;
+    if (lda < std::max(1, n)) return -4;
+    if (!b && n > 0) return -5;
+    if (!x && n > 0) return -6;
+    if (precision < NROCM_PREC_FP64 || precision > NROCM_PREC_BF16X3) return -7;
+    if (!(tol >= 0.0)) return -8;
+    if (maxit < 1) return -9;
+    if (n == 0) {
+        if (iters_host) *iters_host = 0;
+        if (berr_host) *berr_host = 0.0;
+        return NROCM_OK;
+    }
+    if (precision == NROCM_PREC_FP64)
+        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, tol, maxit, iters_host, berr_host);
+    const NrocmTrailing tr = precision == NROCM_PREC_BF16   ? TRAIL_BF16
+                             : precision == NROCM_PREC_BF16X3 ? TRAIL_BF16X3
+                                                              : TRAIL_F32;
+    return gesv_ir<float>(h, n, a, lda, b, x, tr, tol, maxit, iters_host, berr_host);
 }
 
 }  // extern "C"
