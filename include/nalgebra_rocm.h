@@ -147,14 +147,17 @@ int nrocm_dqr_qtmul(nrocm_handle handle, int m, int n, const double *qr, int lda
 int nrocm_dqr_rsolve(nrocm_handle handle, int n, const double *qr, int lda,
                      const double *diag, int nrhs, double *b, int ldb, int *info);
 
-/* Batched tall least squares, min ||A_j x - b_j||, A_j m_j x n (f32, lda m_j),
- * batch created with the row counts m_j (nrhs = 1).  Blocked compact-WY
- * Householder QR with FP32 MFMA trailing updates, then x = R^{-1} Q^T b and
- * `refine` FP64 residual-refinement steps (r = b - A x in FP64, x += R^{-1} Q^T r).
- * A_j and b_j are overwritten (by the factorization and Q^T b); x is n x count
- * f64, info[j] = 0 or 1 + index of a zero R diagonal.  Replaces the tall
- * solve nalgebra-lapack's QrDecomposition::solve provides
- * (nalgebra-lapack/src/qr/abstraction.rs:48-77, qr_util.rs:167-230). */
+/* Batched tall least squares, min ||A_j x - b_j||, A_j m_j x n (f32, lda m_j,
+ * n <= m_j <= 2048, n <= 64), batch created with the row counts m_j.  Blocked
+ * compact-WY Householder QR of a workspace copy of [A_j | b_j] (LAPACK
+ * sgeqrf/slarft conventions) with the WY products V^T C and C - V (T^T W) on
+ * FP32 MFMA, x = R^{-1} (Q^T b)[0..n), then `refine` FP64 residual-refinement
+ * steps x += R^{-1} (Q^T (b - A x))[0..n) with b - A x formed in FP64.  A_j and
+ * b_j are not modified; x is n x count f64 (column j = system j); info[j] = 0 or
+ * 1 + index of a zero R diagonal.  Replaces the tall solve nalgebra-lapack's
+ * QrDecomposition::solve provides (nalgebra-lapack/src/qr/abstraction.rs:48-77,
+ * qr_util.rs:167-230: ormqr then trtrs); core nalgebra's QR::solve is square-only
+ * (src/linalg/qr.rs:214-217). */
 int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *const *a,
                         float *const *b, double *x, int refine, int *info);
 

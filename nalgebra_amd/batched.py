@@ -137,7 +137,7 @@ class LeastSquaresBatch:
         return _pointer_array(A, self.offA), _pointer_array(b, self.offb)
 
     def solve(self, A: torch.Tensor, b: torch.Tensor, refine: int = 1, ptrs=None):
-        """Returns (x: count x n float64, info int32) -- A and b are overwritten."""
+        """Returns (x: count x n float64, info int32); A and b are not modified."""
         if ptrs is None:
             ptrs = self.pointers(A, b)
         x = torch.empty(self.count * self.n, dtype=torch.float64, device=self.device)

@@ -23,17 +23,23 @@ namespace {
 
 // ------------------------------------------------------------------ dispatch tables
 
-using FwdLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
-                         double *const *, double *const *, int, int *, double *, double *, int,
-                         int, int);
+using FacLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
+                         double *const *, int *, double *, int, int);
+using SolLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
+                         double *const *, double *const *, int, const int *, double *, double *,
+                         int, int, int);
 using BwdLaunch = int (*)(unsigned, hipStream_t, const int *, int, int, const int *,
                          const double *const *, const double *const *, const double *const *,
                          double *const *, double *const *, int);
 
-const FwdLaunch kFwd[8] = {nrocm_chol_fwd_launch_8,  nrocm_chol_fwd_launch_16,
-                           nrocm_chol_fwd_launch_24, nrocm_chol_fwd_launch_32,
-                           nrocm_chol_fwd_launch_40, nrocm_chol_fwd_launch_48,
-                           nrocm_chol_fwd_launch_56, nrocm_chol_fwd_launch_64};
+const FacLaunch kFac[8] = {nrocm_chol_potrf_launch_8,  nrocm_chol_potrf_launch_16,
+                           nrocm_chol_potrf_launch_24, nrocm_chol_potrf_launch_32,
+                           nrocm_chol_potrf_launch_40, nrocm_chol_potrf_launch_48,
+                           nrocm_chol_potrf_launch_56, nrocm_chol_potrf_launch_64};
+const SolLaunch kSol[8] = {nrocm_chol_potrs_launch_8,  nrocm_chol_potrs_launch_16,
+                           nrocm_chol_potrs_launch_24, nrocm_chol_potrs_launch_32,
+                           nrocm_chol_potrs_launch_40, nrocm_chol_potrs_launch_48,
+                           nrocm_chol_potrs_launch_56, nrocm_chol_potrs_launch_64};
 const BwdLaunch kBwd[8] = {nrocm_chol_bwd_launch_8,  nrocm_chol_bwd_launch_16,
                            nrocm_chol_bwd_launch_24, nrocm_chol_bwd_launch_32,
                            nrocm_chol_bwd_launch_40, nrocm_chol_bwd_launch_48,
@@ -53,8 +59,13 @@ int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const
         const int items = bin.end - bin.begin;
         if (items <= 0) continue;
         const int mpw = mpw_of_bin(bin.kernel);
-        if (kFwd[bin.kernel](exact, nrocm_cdiv(items, mpw), st, bt->order_dev, bin.begin, bin.end,
-                             bt->n_dev, A, B, bt->nrhs, info, nullptr, nullptr, 0, 0, 0))
+        const unsigned grid = nrocm_cdiv(items, mpw);
+        if (kFac[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A, info, nullptr, 0,
+                             0))
+            NROCM_CHECK_LAUNCH(h);
+        if (bt->nrhs > 0 && B &&
+            kSol[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A, B, bt->nrhs, info,
+                             nullptr, nullptr, 0, 0, 0))
             NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;
@@ -68,8 +79,10 @@ int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int 
                       int *info, bool exact)
 {
     const int bin = std::max(0, (n + 7) / 8 - 1);
-    if (kFwd[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, nullptr, nrhs, info, a, b,
-                  n, lda, ldb))
+    if (kFac[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, info, a, n, lda))
+        NROCM_CHECK_LAUNCH(h);
+    if (b && nrhs > 0 &&
+        kSol[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, nullptr, nrhs, info, a, b, n, lda, ldb))
         NROCM_CHECK_LAUNCH(h);
     return NROCM_OK;
 }

@@ -20,10 +20,13 @@ template <bool EXACT>
 __device__ __forceinline__ double sub_prod(double acc, double x, double y)
 {
     // nalgebra axpy with factor -x: fl(fl(-x*y) + acc) == fl(acc - fl(x*y)).
-    // The library is compiled with -ffp-contract=off, so this stays unfused.
+    // One asm block per update keeps the product next to its subtraction: as
+    // separate IR operations the scheduler hoists every product of a step and
+    // spills the row array to scratch.
     if constexpr (EXACT) {
-        double p = x * y;
-        return acc - p;
+        double r;
+        asm("v_mul_f64 %0, %1, %2\n\tv_add_f64 %0, %3, -%0" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+        return r;
     } else {
         return __builtin_fma(-x, y, acc);
     }
@@ -45,16 +48,12 @@ struct Smem {
     double acc[2][MPW * 8 * CH];  // 8 accumulators per group and right-hand side
 };
 
-// Loads the lower triangle of row r (identity padding) into a[].
+// Loads the lower triangle of row r into a[] (zeros elsewhere).
 template <int NB>
 __device__ __forceinline__ void load_lower(double (&a)[NB], const double *A, int lda, int n, int r)
 {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        double v = (r == c) ? 1.0 : 0.0;
-        if (c <= r && r < n) v = A[r + (size_t)c * lda];
-        a[c] = v;
-    }
+    for (int c = 0; c < NB; ++c) a[c] = (c <= r && r < n) ? A[r + (size_t)c * lda] : 0.0;
 }
 
 template <int NB>
@@ -66,64 +65,35 @@ __device__ __forceinline__ void store_lower(const double (&a)[NB], double *A, in
 }
 
 // Right-looking factorisation in registers; returns 0 or 1 + first failing column.
+// All NB steps run: rows and columns beyond n hold zeros (padding rows can
+// only ever update padding columns, so their NaNs never reach stored values).
+// The failing column is found afterwards: pivot k failed iff its square root
+// is not > 0 (d <= 0 gives NaN or +-0, NaN gives NaN), and nalgebra stops at
+// the first such k.
 template <int NB, int G, bool EXACT>
-__device__ __forceinline__ int factor(double (&a)[NB], Smem<64 / G> &s, int n, int nmax, int g, int r, int lane)
+__device__ __forceinline__ int factor(double (&a)[NB], Smem<64 / G> &s, int n, int g, int r, int lane)
 {
-    int fail = 0;
+    double mydiag = 1.0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        if (k < nmax) {
-            s.col[lane] = a[k];
-            __syncthreads();
-            const double d = s.col[g * G + k];
-            if (!(d > 0.0) && fail == 0 && k < n) fail = k + 1;
-            const double den = sqrt(d);
-            if (r == k) {
-                a[k] = den;
-                s.diag[lane] = den;
-            } else if (r > k) {
-                a[k] = a[k] / den;
-            }
-            // second slot: the divided column (col is still being read above)
-            s.xch[0][lane] = a[k];
-            __syncthreads();
-#pragma unroll
-            for (int j = k + 1; j < NB; ++j) {
-                const double lj = s.xch[0][g * G + j];
-                a[j] = sub_prod<EXACT>(a[j], lj, a[k]);
-            }
+        s.col[lane] = a[k];
+        __syncthreads();
+        const double den = sqrt(s.col[g * G + k]);
+        if (r == k) {
+            a[k] = den;
+            mydiag = den;
+        } else if (r > k) {
+            a[k] = a[k] / den;
         }
+        s.xch[0][lane] = a[k];
+        __syncthreads();
+#pragma unroll
+        for (int j = k + 1; j < NB; ++j) a[j] = sub_prod<EXACT>(a[j], s.xch[0][g * G + j], a[k]);
     }
-    return fail;
-}
-
-// Register element a[i] for a wave-uniform runtime index i.  The switch lowers
-// to uniform scalar branches, so a[] stays in VGPRs (a plain a[i] with runtime
-// i would demote the whole array to scratch memory).
-template <int NB>
-__device__ __forceinline__ double reg_at(const double (&a)[NB], int i)
-{
-    double v = 0.0;
-    switch (i) {
-#define NROCM_RC(k) \
-    case k:         \
-        if constexpr (k < NB) v = a[k]; \
-        break;
-        NROCM_RC(0) NROCM_RC(1) NROCM_RC(2) NROCM_RC(3) NROCM_RC(4) NROCM_RC(5) NROCM_RC(6)
-        NROCM_RC(7) NROCM_RC(8) NROCM_RC(9) NROCM_RC(10) NROCM_RC(11) NROCM_RC(12) NROCM_RC(13)
-        NROCM_RC(14) NROCM_RC(15) NROCM_RC(16) NROCM_RC(17) NROCM_RC(18) NROCM_RC(19) NROCM_RC(20)
-        NROCM_RC(21) NROCM_RC(22) NROCM_RC(23) NROCM_RC(24) NROCM_RC(25) NROCM_RC(26) NROCM_RC(27)
-        NROCM_RC(28) NROCM_RC(29) NROCM_RC(30) NROCM_RC(31) NROCM_RC(32) NROCM_RC(33) NROCM_RC(34)
-        NROCM_RC(35) NROCM_RC(36) NROCM_RC(37) NROCM_RC(38) NROCM_RC(39) NROCM_RC(40) NROCM_RC(41)
-        NROCM_RC(42) NROCM_RC(43) NROCM_RC(44) NROCM_RC(45) NROCM_RC(46) NROCM_RC(47) NROCM_RC(48)
-        NROCM_RC(49) NROCM_RC(50) NROCM_RC(51) NROCM_RC(52) NROCM_RC(53) NROCM_RC(54) NROCM_RC(55)
-        NROCM_RC(56) NROCM_RC(57) NROCM_RC(58) NROCM_RC(59) NROCM_RC(60) NROCM_RC(61) NROCM_RC(62)
-        NROCM_RC(63)
-#undef NROCM_RC
-    default:
-        break;
-    }
-    return v;
+    const bool bad = r < n && !(mydiag > 0.0);
+    unsigned long long m = __ballot(bad);
+    m = (m >> (g * G)) & (G == 64 ? ~0ull : ((1ull << G) - 1));
+    return m ? (int)__builtin_ctzll(m) + 1 : 0;
 }
 
 __device__ __forceinline__ double reg_q(const double (&b)[CH], int q)
@@ -131,149 +101,236 @@ __device__ __forceinline__ double reg_q(const double (&b)[CH], int q)
     return q == 0 ? b[0] : q == 1 ? b[1] : q == 2 ? b[2] : b[3];
 }
 
+constexpr int PF = 8;  // substitution steps whose L elements are loaded ahead
+
 // Forward substitution L y = b (solve.rs:488-519) for CH right-hand sides:
 // coeff = b_i / L_ii; b_i = coeff; b_r = fl(fl(-coeff * L_ri) + b_r) for r > i.
-template <int NB, int G, bool EXACT>
-__device__ __forceinline__ void forward(const double (&a)[NB], double (&b)[CH], Smem<64 / G> &s, int nmax,
-                                        int g, int r, int lane)
+// L is read from memory (the factor kernel's output); the elements of the
+// next PF steps are loaded while the current PF steps run.
+template <int G, bool EXACT>
+__device__ __forceinline__ void forward(const double *L, int lda, double (&b)[CH], Smem<64 / G> &s, int n,
+                                        int nmax, int g, int r, int lane)
 {
-    for (int i = 0; i < nmax; ++i) {
-        const double li = reg_at<NB>(a, i);
-        double *buf = s.xch[i & 1];
+    const bool row_ok = r < n;
+    double cur[PF], nxt[PF];
 #pragma unroll
-        for (int q = 0; q < CH; ++q) buf[lane * CH + q] = b[q];
-        __syncthreads();
-        const double di = s.diag[g * G + i];
+    for (int t = 0; t < PF; ++t) cur[t] = (row_ok && t < nmax && r >= t) ? L[r + (size_t)t * lda] : 0.0;
+    for (int i0 = 0; i0 < nmax; i0 += PF) {
 #pragma unroll
-        for (int q = 0; q < CH; ++q) {
-            const double coeff = buf[(g * G + i) * CH + q] / di;
-            if (r == i) b[q] = coeff;
-            else if (r > i) b[q] = sub_prod<EXACT>(b[q], coeff, li);
+        for (int t = 0; t < PF; ++t) {
+            const int i = i0 + PF + t;
+            nxt[t] = (row_ok && i < nmax && r >= i) ? L[r + (size_t)i * lda] : 0.0;
         }
+#pragma unroll
+        for (int t = 0; t < PF; ++t) {
+            const int i = i0 + t;
+            if (i < nmax) {
+                double *buf = s.xch[i & 1];
+#pragma unroll
+                for (int q = 0; q < CH; ++q) buf[lane * CH + q] = b[q];
+                __syncthreads();
+                const double di = s.diag[g * G + i];
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    const double coeff = buf[(g * G + i) * CH + q] / di;
+                    if (r == i) b[q] = coeff;
+                    else if (r > i) b[q] = sub_prod<EXACT>(b[q], coeff, cur[t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PF; ++t) cur[t] = nxt[t];
     }
     __syncthreads();
+}
+
+// One backward step i with li = L[r, i] (0 above the diagonal / outside).
+template <int G, bool EXACT>
+__device__ __forceinline__ void backward_step(int i, double li, double (&b)[CH], Smem<64 / G> &s, int n, int g,
+                                              int r, int lane)
+{
+    const double di = s.diag[g * G + i];
+    if constexpr (EXACT) {
+        double *p = s.xch[i & 1];
+        double *acc = s.acc[i & 1];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) p[lane * CH + q] = li * b[q];
+        __syncthreads();
+        const int len = n - i - 1;
+        const int full = len & ~7;
+        // Every load of a chain is independent of the adds, so all are issued
+        // at once; out-of-range slots add +0.0, an exact identity here (the
+        // running sums start at +0 and cannot become -0).
+        if (r < 8 && len > 0) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < G / 8; ++m) {
+                    const int e = r + 8 * m;
+                    t += (e < full) ? p[(g * G + i + 1 + e) * CH + q] : 0.0;
+                }
+                acc[(g * 8 + r) * CH + q] = t;
+            }
+        }
+        __syncthreads();
+        if (r == i && i < n) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                double res = 0.0;
+                if (len > 0) {
+                    const double *ac = acc + g * 8 * CH + q;
+                    res += ac[0 * CH] + ac[4 * CH];
+                    res += ac[1 * CH] + ac[5 * CH];
+                    res += ac[2 * CH] + ac[6 * CH];
+                    res += ac[3 * CH] + ac[7 * CH];
+#pragma unroll
+                    for (int m = 0; m < 7; ++m) {
+                        const int e = full + m;
+                        res += (e < len) ? p[(g * G + i + 1 + e) * CH + q] : 0.0;
+                    }
+                }
+                b[q] = (b[q] - res) / di;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            double v = (r > i && r < n) ? li * b[q] : 0.0;
+            v = group_sum<G>(v);
+            if (r == i && i < n) b[q] = (b[q] - v) / di;
+        }
+    }
 }
 
 // Backward substitution L^T x = y (solve.rs:697-755) for CH right-hand sides:
 // b_i = (b_i - dotc(L[i+1..,i], b[i+1..])) / L_ii.  EXACT reproduces dotx's
 // eight accumulators (acc_t sums positions t, t+8, ... of the full blocks of 8,
 // then 0 + (acc0+acc4) + (acc1+acc5) + (acc2+acc6) + (acc3+acc7), then the tail).
-template <int NB, int G, bool EXACT>
-__device__ __forceinline__ void backward(const double (&a)[NB], double (&b)[CH], Smem<64 / G> &s, int n,
+template <int G, bool EXACT>
+__device__ __forceinline__ void backward(const double *L, int lda, double (&b)[CH], Smem<64 / G> &s, int n,
                                          int nmax, int g, int r, int lane)
 {
-    for (int i = nmax - 1; i >= 0; --i) {
-        const double li = reg_at<NB>(a, i);
-        if constexpr (EXACT) {
-            double *p = s.xch[i & 1];
-            double *acc = s.acc[i & 1];
+    const bool row_ok = r < n;
+    double cur[PF], nxt[PF];
+    // blocks of PF steps counted down from nmax - 1
 #pragma unroll
-            for (int q = 0; q < CH; ++q) p[lane * CH + q] = li * b[q];
-            __syncthreads();
-            const int len = n - i - 1;
-            const int full = len & ~7;
-            if (r < 8 && len > 0) {
+    for (int t = 0; t < PF; ++t) {
+        const int i = nmax - 1 - t;
+        cur[t] = (row_ok && i >= 0 && r >= i) ? L[r + (size_t)i * lda] : 0.0;
+    }
+    for (int i0 = nmax - 1; i0 >= 0; i0 -= PF) {
 #pragma unroll
-                for (int q = 0; q < CH; ++q) {
-                    double t = 0.0;
-                    for (int e = r; e < full; e += 8) t += p[(g * G + i + 1 + e) * CH + q];
-                    acc[(g * 8 + r) * CH + q] = t;
-                }
-            }
-            __syncthreads();
-            if (r == i && i < n) {
-#pragma unroll
-                for (int q = 0; q < CH; ++q) {
-                    double res = 0.0;
-                    if (len > 0) {
-                        const double *ac = acc + g * 8 * CH + q;
-                        res += ac[0 * CH] + ac[4 * CH];
-                        res += ac[1 * CH] + ac[5 * CH];
-                        res += ac[2 * CH] + ac[6 * CH];
-                        res += ac[3 * CH] + ac[7 * CH];
-                        for (int e = full; e < len; ++e) res += p[(g * G + i + 1 + e) * CH + q];
-                    }
-                    b[q] = (b[q] - res) / li;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < CH; ++q) {
-                double v = (r > i && r < n) ? li * b[q] : 0.0;
-                v = group_sum<G>(v);
-                if (r == i && i < n) b[q] = (b[q] - v) / li;
-            }
+        for (int t = 0; t < PF; ++t) {
+            const int i = i0 - PF - t;
+            nxt[t] = (row_ok && i >= 0 && r >= i) ? L[r + (size_t)i * lda] : 0.0;
         }
+#pragma unroll
+        for (int t = 0; t < PF; ++t) {
+            const int i = i0 - t;
+            if (i >= 0) backward_step<G, EXACT>(i, cur[t], b, s, n, g, r, lane);
+        }
+#pragma unroll
+        for (int t = 0; t < PF; ++t) cur[t] = nxt[t];
     }
     __syncthreads();
 }
 
-// One launch per size bin; matrices order[begin..end) (sorted by decreasing n).
-// With order == nullptr a single matrix (A1, B1, n1, lda1, ldb1) is processed.
-template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu<NB>(), 8))) void potrf_potrs_kernel(
-    const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
-    double *const *__restrict__ Aptr, double *const *__restrict__ Bptr, int nrhs,
-    int *__restrict__ info, double *A1, double *B1, int n1, int lda1, int ldb1)
+// Member lookup shared by the kernels: order[begin..end) (sorted by decreasing
+// n), or with order == nullptr the single matrix (A1, B1, n1, lda1, ldb1).
+struct Member {
+    int jm, n, lda, ldb;
+    double *A, *B;
+    bool valid;
+};
+
+template <int G>
+__device__ __forceinline__ Member member(const int *order, int begin, int end, const int *nsz, double *const *Aptr,
+                                         double *const *Bptr, double *A1, double *B1, int n1, int lda1, int ldb1,
+                                         int g)
 {
     constexpr int MPW = 64 / G;
-    __shared__ Smem<MPW> s;
-    const int lane = threadIdx.x;
-    const int g = lane / G, r = lane % G;
-    const bool has_b = order ? (Bptr != nullptr) : (B1 != nullptr);  // uniform
-    int jm = 0, n = 0, lda = 0, ldb = 0;
-    double *A = nullptr, *B = nullptr;
-    bool valid;
+    Member mb{0, 0, 0, 0, nullptr, nullptr, false};
     if (order) {
         const int item = begin + (int)blockIdx.x * MPW + g;
-        valid = item < end;
-        if (valid) {
-            jm = order[item];
-            n = nsz[jm];
-            A = Aptr[jm];
-            B = Bptr ? Bptr[jm] : nullptr;
-            lda = ldb = n;
+        mb.valid = item < end;
+        if (mb.valid) {
+            mb.jm = order[item];
+            mb.n = nsz[mb.jm];
+            mb.A = Aptr[mb.jm];
+            mb.B = Bptr ? Bptr[mb.jm] : nullptr;
+            mb.lda = mb.ldb = mb.n;
         }
     } else {
-        valid = g == 0;
-        if (valid) {
-            n = n1;
-            A = A1;
-            B = B1;
-            lda = lda1;
-            ldb = ldb1;
+        mb.valid = g == 0;
+        if (mb.valid) {
+            mb.n = n1;
+            mb.A = A1;
+            mb.B = B1;
+            mb.lda = lda1;
+            mb.ldb = ldb1;
         }
     }
+    return mb;
+}
+
+// Factorisation: one launch per size bin, the row array in registers.
+template <int NB, int G, bool EXACT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu<NB>(), 8))) void potrf_kernel(
+    const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
+    double *const *__restrict__ Aptr, int *__restrict__ info, double *A1, int n1, int lda1)
+{
+    __shared__ Smem<64 / G> s;
+    const int lane = threadIdx.x;
+    const int g = lane / G, r = lane % G;
+    const Member mb = member<G>(order, begin, end, nsz, Aptr, nullptr, A1, nullptr, n1, lda1, 0, g);
     // Members are sorted by decreasing size, so group 0 holds the largest n.
-    const int nmax = __builtin_amdgcn_readfirstlane(n);
-
+    const int nmax = __builtin_amdgcn_readfirstlane(mb.n);
     double a[NB];
-    load_lower<NB>(a, A, lda, n, r);
-    const int fail = factor<NB, G, EXACT>(a, s, n, nmax, g, r, lane);
-    store_lower<NB>(a, A, lda, n, r);
+    load_lower<NB>(a, mb.A, mb.lda, mb.n, r);
+    (void)nmax;
+    const int fail = factor<NB, G, EXACT>(a, s, mb.n, g, r, lane);
+    store_lower<NB>(a, mb.A, mb.lda, mb.n, r);
+    if (mb.valid && r == 0 && info) info[order ? mb.jm : 0] = fail;
+}
 
-    if (has_b && nrhs > 0) {
-        for (int c0 = 0; c0 < nrhs; c0 += CH) {
-            double b[CH];
+// Substitutions (Cholesky::solve_mut) with L read from memory: few registers,
+// so many wavefronts hide the latency of the step-by-step dependence chains.
+// Members whose factorisation failed keep their right-hand sides.
+template <int NB, int G, bool EXACT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void potrs_kernel(
+    const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
+    double *const *__restrict__ Aptr, double *const *__restrict__ Bptr, int nrhs, const int *__restrict__ info,
+    double *A1, double *B1, int n1, int lda1, int ldb1)
+{
+    __shared__ Smem<64 / G> s;
+    const int lane = threadIdx.x;
+    const int g = lane / G, r = lane % G;
+    const Member mb = member<G>(order, begin, end, nsz, Aptr, Bptr, A1, B1, n1, lda1, ldb1, g);
+    const int nmax = __builtin_amdgcn_readfirstlane(mb.n);
+    const int fail = mb.valid ? info[order ? mb.jm : 0] : 1;
+    const int n = mb.n;
+    s.diag[lane] = (r < n) ? mb.A[r + (size_t)r * mb.lda] : 1.0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nrhs; c0 += CH) {
+        double b[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            b[q] = (r < n && c0 + q < nrhs && fail == 0) ? mb.B[r + (size_t)(c0 + q) * mb.ldb] : 0.0;
+        forward<G, EXACT>(mb.A, mb.lda, b, s, n, nmax, g, r, lane);
+        backward<G, EXACT>(mb.A, mb.lda, b, s, n, nmax, g, r, lane);
+        if (fail == 0 && r < n) {
 #pragma unroll
             for (int q = 0; q < CH; ++q)
-                b[q] = (valid && r < n && c0 + q < nrhs && fail == 0) ? B[r + (size_t)(c0 + q) * ldb] : 0.0;
-            forward<NB, G, EXACT>(a, b, s, nmax, g, r, lane);
-            backward<NB, G, EXACT>(a, b, s, n, nmax, g, r, lane);
-            if (valid && fail == 0 && r < n) {
-#pragma unroll
-                for (int q = 0; q < CH; ++q)
-                    if (c0 + q < nrhs) B[r + (size_t)(c0 + q) * ldb] = b[q];
-            }
+                if (c0 + q < nrhs) mb.B[r + (size_t)(c0 + q) * mb.ldb] = b[q];
         }
     }
-    if (valid && r == 0 && info) info[order ? jm : 0] = fail;
 }
 
 // Adjoint of X = A^{-1} B given L: Lambda = (L L^T)^{-1} dX (same arithmetic as
 // Cholesky::solve_mut), dB = Lambda, dA = -1/2 (Lambda X^T + X Lambda^T).
 template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu<NB>(), 8))) void potrs_backward_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void potrs_backward_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
     const double *const *__restrict__ Lptr, const double *const *__restrict__ Xptr,
     const double *const *__restrict__ dXptr, double *const *__restrict__ dAptr,
@@ -295,12 +352,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu
     double *dB = valid ? dBptr[jm] : nullptr;
     const int nmax = __builtin_amdgcn_readfirstlane(n);
 
-    double a[NB];
-    load_lower<NB>(a, L, n, n, r);
-    // the diagonal of L is a[r] of lane r; publish it for forward()
-#pragma unroll
-    for (int k = 0; k < NB; ++k)
-        if (r == k) s.diag[lane] = a[k];
+    // the diagonal of L, published for the substitutions; padding rows get 1
+    s.diag[lane] = (r < n) ? L[r + (size_t)r * n] : 1.0;
     __syncthreads();
 
     for (int c0 = 0; c0 < nrhs; c0 += CH) {
@@ -311,8 +364,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu
             b[q] = in ? dX[r + (size_t)(c0 + q) * n] : 0.0;
             x[q] = in ? X[r + (size_t)(c0 + q) * n] : 0.0;
         }
-        forward<NB, G, EXACT>(a, b, s, nmax, g, r, lane);
-        backward<NB, G, EXACT>(a, b, s, n, nmax, g, r, lane);
+        forward<G, EXACT>(L, n, b, s, n, nmax, g, r, lane);
+        backward<G, EXACT>(L, n, b, s, n, nmax, g, r, lane);
         if (r < n) {
 #pragma unroll
             for (int q = 0; q < CH; ++q)
@@ -347,29 +400,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu
 }  // namespace nrocm_chol
 
 #define NROCM_CHOL_DECLARE(NB)                                                                    \
-    int nrocm_chol_fwd_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order,   \
-                                   int begin, int end, const int *nsz, double *const *A,          \
-                                   double *const *B, int nrhs, int *info, double *A1, double *B1, \
-                                   int n1, int lda1, int ldb1);                                   \
+    int nrocm_chol_potrf_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     int *info, double *A1, int n1, int lda1);                    \
+    int nrocm_chol_potrs_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     double *const *B, int nrhs, const int *info, double *A1,     \
+                                     double *B1, int n1, int lda1, int ldb1);                     \
     int nrocm_chol_bwd_launch_##NB(unsigned grid, hipStream_t st, const int *order, int begin,    \
                                    int end, const int *nsz, const double *const *L,               \
                                    const double *const *X, const double *const *dX,               \
                                    double *const *dA, double *const *dB, int nrhs);
 
 #define NROCM_CHOL_DEFINE(NB)                                                                     \
-    int nrocm_chol_fwd_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order,   \
-                                   int begin, int end, const int *nsz, double *const *A,          \
-                                   double *const *B, int nrhs, int *info, double *A1, double *B1, \
-                                   int n1, int lda1, int ldb1)                                    \
+    int nrocm_chol_potrf_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     int *info, double *A1, int n1, int lda1)                     \
     {                                                                                             \
         using namespace nrocm_chol;                                                               \
         constexpr int G = group_for<NB>();                                                        \
         if (exact)                                                                                \
-            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st,    \
-                               order, begin, end, nsz, A, B, nrhs, info, A1, B1, n1, lda1, ldb1); \
+            hipLaunchKernelGGL((potrf_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st, order,   \
+                               begin, end, nsz, A, info, A1, n1, lda1);                           \
         else                                                                                      \
-            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st,   \
-                               order, begin, end, nsz, A, B, nrhs, info, A1, B1, n1, lda1, ldb1); \
+            hipLaunchKernelGGL((potrf_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st, order,  \
+                               begin, end, nsz, A, info, A1, n1, lda1);                           \
+        return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
+    }                                                                                             \
+    int nrocm_chol_potrs_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     double *const *B, int nrhs, const int *info, double *A1,     \
+                                     double *B1, int n1, int lda1, int ldb1)                      \
+    {                                                                                             \
+        using namespace nrocm_chol;                                                               \
+        constexpr int G = group_for<NB>();                                                        \
+        if (exact)                                                                                \
+            hipLaunchKernelGGL((potrs_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st, order,   \
+                               begin, end, nsz, A, B, nrhs, info, A1, B1, n1, lda1, ldb1);        \
+        else                                                                                      \
+            hipLaunchKernelGGL((potrs_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st, order,  \
+                               begin, end, nsz, A, B, nrhs, info, A1, B1, n1, lda1, ldb1);        \
         return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
     }                                                                                             \
     int nrocm_chol_bwd_launch_##NB(unsigned grid, hipStream_t st, const int *order, int begin,    \
