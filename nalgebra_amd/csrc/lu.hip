@@ -147,15 +147,13 @@ int getrf_impl(nrocm_handle h, hipStream_t st, int m, int n, T *A, int lda, int 
             const PanelCfg<T> cfg = panel_cfg<T>(m - b0);
             const int w = std::min(cfg.w, j1 - b0);
             LU_TRY(launch_panel<T>(st, cfg.rpt, cfg.w, A, lda, m, b0, w, ipiv, info));
-            LU_TRY(laswp<T>(st, A, lda, j0, b0, ipiv, b0, b0 + w));
-            LU_TRY(laswp<T>(st, A, lda, b0 + w, j1, ipiv, b0, b0 + w));
-            if (b0 + w < j1) {
-                LU_TRY(trsm<T>(st, A, lda, b0, w, b0 + w, j1, true));
-                // intra-panel update is always exact-order (it is tiny)
-                T *C = A + (b0 + w) + (size_t)(b0 + w) * lda;
-                LU_TRY((simt_update<T, true, false>(st, C, lda, m - b0 - w, j1 - b0 - w,
-                                                    A + (b0 + w) + (size_t)b0 * lda, lda,
-                                                    A + b0 + (size_t)(b0 + w) * lda, 1, lda, w, 0)));
+            // swaps for panel columns [j0, b0); swaps + U12 + rank-w update for
+            // [b0+w, j1) -- always in exact order (the panel is narrow)
+            const int ncols = (b0 - j0) + (j1 - b0 - w);
+            if (ncols > 0) {
+                hipLaunchKernelGGL((panel_cols_kernel<T>), dim3(ncols), dim3(256), 0, st, A, lda, m, b0, w, j0,
+                                   b0 + w, j1, ipiv, 1);
+                LU_TRY(hipGetLastError());
             }
             b0 += w;
         }
@@ -358,16 +356,31 @@ __global__ __launch_bounds__(256) void potrs_kernel(const double *L, int lda, in
 
 // ---------------------------------------------------------------- internal entry points
 
+// The blocked factorisation is thousands of small launches at large n, so the
+// sequence is captured once per (shape, buffers, flavour) and replayed.
+template <typename T>
+int getrf_graph(nrocm_handle h, int m, int n, T *A, int lda, int *ipiv, int *info, NrocmTrailing trail)
+{
+    if (std::min(m, n) <= 64) return getrf_impl<T>(h, h->stream, m, n, A, lda, ipiv, info, trail);
+    const long long key[8] = {1000 + (long long)sizeof(T), m, n, lda, (long long)A, (long long)ipiv,
+                              (long long)info, (long long)trail};
+    return nrocm_graph_run(h, key, [&](hipStream_t st) {
+        return getrf_impl<T>(h, st, m, n, A, lda, ipiv, info, trail);
+    });
+}
+
 int nrocm_getrf_d(nrocm_handle h, hipStream_t st, int m, int n, double *A, int lda, int *ipiv, int *info,
                   NrocmTrailing trail)
 {
-    return getrf_impl<double>(h, st, m, n, A, lda, ipiv, info, trail);
+    (void)st;
+    return getrf_graph<double>(h, m, n, A, lda, ipiv, info, trail);
 }
 
 int nrocm_getrf_s(nrocm_handle h, hipStream_t st, int m, int n, float *A, int lda, int *ipiv, int *info,
                   NrocmTrailing trail)
 {
-    return getrf_impl<float>(h, st, m, n, A, lda, ipiv, info, trail);
+    (void)st;
+    return getrf_graph<float>(h, m, n, A, lda, ipiv, info, trail);
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -384,8 +397,7 @@ int nrocm_dgetrf(nrocm_handle h, int m, int n, double *a, int lda, int *ipiv, in
     if (!ipiv && std::min(m, n) > 0) return -6;
     if (!info) return -7;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -8;
-    return getrf_impl<double>(h, h->stream, m, n, a, lda, ipiv, info,
-                              mode == NROCM_EXACT ? TRAIL_EXACT : TRAIL_F64);
+    return getrf_graph<double>(h, m, n, a, lda, ipiv, info, mode == NROCM_EXACT ? TRAIL_EXACT : TRAIL_F64);
 }
 
 int nrocm_dgetrs(nrocm_handle h, int n, int nrhs, const double *lu, int lda, const int *ipiv, double *b, int ldb,

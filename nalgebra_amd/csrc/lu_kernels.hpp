@@ -247,6 +247,70 @@ __global__ __launch_bounds__(256) void update_kernel(T *C, int ldc, int m, int n
         }
 }
 
+// For each column c of [c0, c1) (one workgroup each) after the base block
+// [b0, b0+w) has been factored: apply its w row swaps to the column; when
+// c >= b0 + w also form U12 (rows b0..b0+w, unit-lower solve with L11) and
+// apply the rank-w update to rows b0+w..m.  Every element's terms are applied
+// in increasing k (EXACT: fl(a - fl(u*l))), as nalgebra's gauss steps do.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_cols_kernel(T *A, int lda, int m, int b0, int w, int cl, int cr0,
+                                                         int cr1, const int *ipiv, int exact)
+{
+    __shared__ int src[64];      // touched row positions
+    __shared__ T val[64];
+    __shared__ int nt;
+    __shared__ T U[64];
+    const int t = threadIdx.x;
+    const int idx = blockIdx.x;
+    // columns [cl, b0) swap only; columns [cr0, cr1) swap + solve + update
+    const int c = idx < b0 - cl ? cl + idx : cr0 + (idx - (b0 - cl));
+    const bool full = c >= cr0;
+    T *col = A + (size_t)c * lda;
+    // ---- compose the w swaps on the touched rows (positions b0..b0+w-1 and their partners)
+    if (t == 0) {
+        int n_ = 0;
+        for (int i = 0; i < w; ++i) {
+            const int p = ipiv[b0 + i];
+            int a = -1, b = -1;
+            for (int q = 0; q < n_; ++q) {
+                if (src[q] == b0 + i) a = q;
+                if (src[q] == p) b = q;
+            }
+            if (a < 0) { a = n_; src[n_] = b0 + i; val[n_] = col[b0 + i]; ++n_; }
+            if (b < 0) { b = n_; src[n_] = p; val[n_] = col[p]; ++n_; }
+            if (a != b) {
+                const T tv = val[a];
+                val[a] = val[b];
+                val[b] = tv;
+            }
+        }
+        nt = n_;
+    }
+    __syncthreads();
+    if (t < nt) col[src[t]] = val[t];
+    __syncthreads();
+    if (!full) return;
+    // ---- U12 for this column: rows b0..b0+w, unit lower L11 (one wavefront)
+    if (t < 64) {
+        T v = t < w ? col[b0 + t] : T(0);
+        for (int i = 0; i < w - 1; ++i) {
+            const T u = __shfl(v, i);
+            if (t > i && t < w) v = sub_prod(v, u, A[(b0 + t) + (size_t)(b0 + i) * lda], exact != 0);
+        }
+        if (t < w) {
+            col[b0 + t] = v;
+            U[t] = v;
+        }
+    }
+    __syncthreads();
+    // ---- rank-w update of rows b0+w..m
+    for (int r = b0 + w + t; r < m; r += blockDim.x) {
+        T a = col[r];
+        for (int q = 0; q < w; ++q) a = sub_prod(a, U[q], A[r + (size_t)(b0 + q) * lda], exact != 0);
+        col[r] = a;
+    }
+}
+
 __global__ void fill_int_kernel(int *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

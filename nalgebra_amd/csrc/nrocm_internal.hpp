@@ -21,7 +21,22 @@ struct nrocm_handle_s {
     size_t ws_bytes = 0;
     // Small device-side scalars (counters, flags, norms).
     void *small = nullptr;          // 4 KiB
+
+    // hipGraph cache for long launch sequences (keyed by every argument that
+    // shapes the sequence), captured on a private stream, replayed on `stream`.
+    struct Graph {
+        long long key[8];
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    hipStream_t capture_stream = nullptr;
 };
+
+// Runs `body(stream)` through the handle's graph cache: on a miss the launches
+// are captured into a new graph; either way the graph is launched on h->stream.
+template <typename F>
+int nrocm_graph_run(nrocm_handle h, const long long (&key)[8], F body);
 
 struct nrocm_batch_s {
     nrocm_handle handle = nullptr;
@@ -67,3 +82,47 @@ int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr);
     } while (0)
 
 static inline unsigned nrocm_cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+template <typename F>
+int nrocm_graph_run(nrocm_handle h, const long long (&key)[8], F body)
+{
+    nrocm_handle_s::Graph *hit = nullptr;
+    for (auto &g : h->graphs) {
+        bool same = true;
+        for (int i = 0; i < 8; ++i) same = same && g.key[i] == key[i];
+        if (same) hit = &g;
+    }
+    if (!hit) {
+        if (h->graphs.size() >= 32) {
+            NROCM_CHECK_HIP(h, hipStreamSynchronize(h->stream));
+            NROCM_CHECK_HIP(h, hipGraphExecDestroy(h->graphs.front().exec));
+            NROCM_CHECK_HIP(h, hipGraphDestroy(h->graphs.front().graph));
+            h->graphs.erase(h->graphs.begin());
+        }
+        if (!h->capture_stream)
+            NROCM_CHECK_HIP(h, hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking));
+        NROCM_CHECK_HIP(h, hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal));
+        const int st = body(h->capture_stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->capture_stream, &graph);
+        if (st != NROCM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return nrocm_fail_hip(h, e, "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return nrocm_fail_hip(h, e2, "hipGraphInstantiate");
+        }
+        nrocm_handle_s::Graph g;
+        for (int i = 0; i < 8; ++i) g.key[i] = key[i];
+        g.graph = graph;
+        g.exec = exec;
+        h->graphs.push_back(g);
+        hit = &h->graphs.back();
+    }
+    NROCM_CHECK_HIP(h, hipGraphLaunch(hit->exec, h->stream));
+    return NROCM_OK;
+}

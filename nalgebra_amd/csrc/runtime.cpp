@@ -70,6 +70,11 @@ int nrocm_destroy(nrocm_handle h)
 {
     if (!h) return -1;
     (void)hipStreamSynchronize(h->stream);
+    for (auto &g : h->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
     if (h->ws) (void)hipFree(h->ws);
     if (h->small) (void)hipFree(h->small);
     delete h;
