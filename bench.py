@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--mode", choices=["exact", "fast"], default="exact")
-    p.add_argument("--workload", choices=["chol", "lu"], default="chol")
+    p.add_argument("--workload", choices=["chol", "lu", "adjoint", "lsq"], default="chol")
     p.add_argument("--count", type=int, default=65536)
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
@@ -260,6 +260,158 @@ def bench_lu(args):
     return out
 
 
+def bench_adjoint(args):
+    """Config 4: 4,096 SPD 32x32 systems, b ~ N(0,1), loss = sum |x|^2; one step =
+    forward (Cholesky::new + solve_mut) + hand-written adjoint (dA, db)."""
+    import torch
+    import nalgebra_amd as nl
+    from nalgebra_amd import _lib
+    lib = _lib.lib()
+    count, n = 4096, 32
+    ns = np.full(count, n, dtype=np.int32)
+    A, offA, B, offB = gen_spd(lib, 4, ns, 1)
+    dev = torch.device("cuda:0")
+    K = max(1, args.steps)
+    A0 = torch.from_numpy(A).to(dev)
+    B0 = torch.from_numpy(B).to(dev)
+    copies = [(A0.clone(), B0.clone()) for _ in range(K)]
+    bt = nl.CholeskyBatch(ns, 1, device=dev)
+    ptrs = [bt.pointers(a, b) for a, b in copies]
+    infos = [torch.empty(count, dtype=torch.int32, device=dev) for _ in range(K)]
+    dX = [torch.empty_like(B0) for _ in range(K)]
+    dA = [torch.empty_like(A0) for _ in range(K)]
+    dB = [torch.empty_like(B0) for _ in range(K)]
+    from nalgebra_amd.batched import _pointer_array
+    bptrs = [(ptrs[i][0], ptrs[i][1], _pointer_array(dX[i], bt.offB), _pointer_array(dA[i], bt.offA),
+              _pointer_array(dB[i], bt.offB)) for i in range(K)]
+    mode = nl.EXACT if args.mode == "exact" else nl.FAST
+
+    def step(i):
+        bt.solve_ptrs(ptrs[i][0], ptrs[i][1], infos[i], mode)
+        torch.mul(copies[i][1], 2.0, out=dX[i])          # dLoss/dx = 2x
+        bt.backward_raw(*bptrs[i])
+
+    for i in range(max(args.warmup, K)):
+        step(i % K)
+    torch.cuda.synchronize()
+    for a, b in copies:
+        a.copy_(A0)
+        b.copy_(B0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        step(i)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / K
+    # parity on a sample against the oracle's forward + analytic adjoint
+    from oracle import oracle as orc
+    Lg = copies[-1][0].cpu().numpy(); Xg = copies[-1][1].cpu().numpy()
+    dAg = dA[-1].cpu().numpy(); dBg = dB[-1].cpu().numpy()
+    bit = True
+    for j in range(0, count, 64):
+        Lr, _ = orc.potrf(A[offA[j]:offA[j] + n * n].reshape(n, n, order="F"))
+        xr = orc.potrs(Lr, B[offB[j]:offB[j] + n])[:, 0]
+        dAr, dBr = orc.potrs_adjoint(Lr, xr, 2 * xr)
+        bit &= bool(np.array_equal(dAg[offA[j]:offA[j] + n * n], dAr.ravel(order="F")))
+        bit &= bool(np.array_equal(dBg[offB[j]:offB[j] + n], dBr))
+    alg_bytes = count * 8 * (n * (n + 1) + 2 * n + n * (n + 1) // 2 + 3 * n + n * n)
+    out = {"metric": METRIC, "value": count / (ms * 1e-3), "unit": "matrices/s (forward+backward)", "n_gpus": 1,
+           "steps": K, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "cholesky_solve_adjoint", "count": count, "n": n, "mode": args.mode},
+           "roofline": {"bound": "hbm", "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "note": "the 54 MB working set is Infinity-Cache resident; launch/latency bound"},
+           "parity": {"sampled_members": count // 64, "bit_identical_vs_oracle_adjoint": bit}}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 5.0:
+            for j in range(count):
+                Lr, _ = orc.potrf(A[offA[j]:offA[j] + n * n].reshape(n, n, order="F"))
+                xr = orc.potrs(Lr, B[offB[j]:offB[j] + n])[:, 0]
+                orc.potrs_adjoint(Lr, xr, 2 * xr)
+                reps += 1
+                if time.perf_counter() - t0 > 5.0:
+                    break
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": reps / tc, "unit": "matrices/s (forward+backward)", "cores": 1,
+                               "kind": "port", "sample": f"{reps} members through the oracle (ctypes per call), "
+                                                         f"{tc:.1f} s"}
+    return out
+
+
+def bench_lsq(args):
+    """Config 5: 2,048 tall f32 systems, m uniform in [256, 2048], n = 64, entries
+    N(0,1), b = A x_true + 1e-3 N(0,1); blocked WY QR on FP32 MFMA + one FP64
+    refinement step; value = systems/s; error vs FP64 LAPACK lstsq on a sample."""
+    import torch
+    import nalgebra_amd as nl
+    from nalgebra_amd import _lib
+    lib = _lib.lib()
+    count, n = 2048, 64
+    ms_ = draw_sizes(lib, 5, count, 256, 2048)
+    m64 = ms_.astype(np.int64)
+    tot = int(m64.sum())
+    Ah = np.zeros(tot * n)
+    lib.nrocm_gen_normal(5, tot * n, Ah.ctypes.data_as(C.c_void_p))
+    xt = np.zeros(count * n)
+    lib.nrocm_gen_normal(6, count * n, xt.ctypes.data_as(C.c_void_p))
+    noise = np.zeros(tot)
+    lib.nrocm_gen_normal(7, tot, noise.ctypes.data_as(C.c_void_p))
+    offA = np.concatenate([[0], np.cumsum(m64 * n)[:-1]])
+    offb = np.concatenate([[0], np.cumsum(m64)[:-1]])
+    bh = np.zeros(tot)
+    for j in range(count):
+        Aj = Ah[offA[j]:offA[j] + ms_[j] * n].reshape(ms_[j], n, order="F")
+        bh[offb[j]:offb[j] + ms_[j]] = Aj @ xt[j * n:(j + 1) * n] + 1e-3 * noise[offb[j]:offb[j] + ms_[j]]
+    A32 = Ah.astype(np.float32); b32 = bh.astype(np.float32)
+    dev = torch.device("cuda:0")
+    Ad = torch.from_numpy(A32).to(dev); bd = torch.from_numpy(b32).to(dev)
+    bt = nl.LeastSquaresBatch(ms_, n, device=dev)
+    ptrs = bt.pointers(Ad, bd)
+    x = torch.empty(count * n, dtype=torch.float64, device=dev)
+    info = torch.empty(count, dtype=torch.int32, device=dev)
+    for _ in range(max(1, args.warmup)):
+        bt.solve_ptrs(ptrs[0], ptrs[1], x, info, 1)
+    torch.cuda.synchronize()
+    K = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        bt.solve_ptrs(ptrs[0], ptrs[1], x, info, 1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / K
+    xg = x.view(count, n).cpu().numpy()
+    worst = 0.0
+    for j in range(0, count, 64):
+        Aj = A32[offA[j]:offA[j] + ms_[j] * n].reshape(ms_[j], n, order="F").astype(np.float64)
+        xr = np.linalg.lstsq(Aj, b32[offb[j]:offb[j] + ms_[j]].astype(np.float64), rcond=None)[0]
+        worst = max(worst, float(np.linalg.norm(xg[j] - xr) / np.linalg.norm(xr)))
+    flops = float((2 * m64 * n * n - 2 * n ** 3 / 3).sum())
+    out = {"metric": METRIC, "value": count / (ms * 1e-3), "unit": "systems/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64 refinement", "data": "synthetic",
+           "config": {"workload": "batched_qr_least_squares", "count": count, "m_range": [256, 2048], "n": n,
+                      "refine": 1},
+           "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": flops / (ms * 1e-3) / 1e12 / 157.3, "traffic": None},
+           "error_vs_fp64_lstsq": {"sampled_systems": count // 64, "max_rel": worst}}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < 10.0 and done < count:
+            j = done
+            orc.qr_ls(A32[offA[j]:offA[j] + ms_[j] * n].reshape(ms_[j], n, order="F"),
+                      b32[offb[j]:offb[j] + ms_[j]])
+            done += 1
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / tc, "unit": "systems/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} systems through the oracle's f32 QR::new + q_tr_mul + "
+                                         f"upper solve, {tc:.1f} s"}
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -269,8 +421,12 @@ def main():
         sys.exit(2)
     if args.workload == "chol":
         out = bench_chol(args)
-    else:
+    elif args.workload == "lu":
         out = bench_lu(args)
+    elif args.workload == "adjoint":
+        out = bench_adjoint(args)
+    else:
+        out = bench_lsq(args)
     print(json.dumps(out), flush=True)
 
 
