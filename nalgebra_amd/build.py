@@ -32,6 +32,9 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
+    # the small-matrix kernels keep whole rows in registers and need their
+    # step loops fully unrolled
+    "-mllvm", "-pragma-unroll-threshold=200000",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]

@@ -31,6 +31,8 @@ using SolLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, co
 using BwdLaunch = int (*)(unsigned, hipStream_t, const int *, int, int, const int *,
                          const double *const *, const double *const *, const double *const *,
                          double *const *, double *const *, int);
+using FusedLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
+                           double *const *, double *const *, int, int *);
 
 const FacLaunch kFac[8] = {nrocm_chol_potrf_launch_8,  nrocm_chol_potrf_launch_16,
                            nrocm_chol_potrf_launch_24, nrocm_chol_potrf_launch_32,
@@ -44,6 +46,11 @@ const BwdLaunch kBwd[8] = {nrocm_chol_bwd_launch_8,  nrocm_chol_bwd_launch_16,
                            nrocm_chol_bwd_launch_24, nrocm_chol_bwd_launch_32,
                            nrocm_chol_bwd_launch_40, nrocm_chol_bwd_launch_48,
                            nrocm_chol_bwd_launch_56, nrocm_chol_bwd_launch_64};
+
+const FusedLaunch kFused[8] = {nrocm_chol_fused_launch_8,  nrocm_chol_fused_launch_16,
+                               nrocm_chol_fused_launch_24, nrocm_chol_fused_launch_32,
+                               nrocm_chol_fused_launch_40, nrocm_chol_fused_launch_48,
+                               nrocm_chol_fused_launch_56, nrocm_chol_fused_launch_64};
 
 int mpw_of_bin(int bin)
 {
@@ -60,12 +67,8 @@ int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const
         if (items <= 0) continue;
         const int mpw = mpw_of_bin(bin.kernel);
         const unsigned grid = nrocm_cdiv(items, mpw);
-        if (kFac[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A, info, nullptr, 0,
-                             0))
-            NROCM_CHECK_LAUNCH(h);
-        if (bt->nrhs > 0 && B &&
-            kSol[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A, B, bt->nrhs, info,
-                             nullptr, nullptr, 0, 0, 0))
+        if (kFused[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
+                               bt->nrhs > 0 ? B : nullptr, bt->nrhs, info))
             NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;

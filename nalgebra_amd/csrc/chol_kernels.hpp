@@ -48,20 +48,43 @@ struct Smem {
     double acc[2][MPW * 8 * CH];  // 8 accumulators per group and right-hand side
 };
 
-// Loads the lower triangle of row r into a[] (zeros elsewhere).
+// A zero that masked-off lanes load instead of branching around their loads.
+__device__ const double kZeroCell[2] = {0.0, 0.0};
+
+// Loads the lower triangle of row r into a[] (zeros elsewhere).  Every load is
+// issued unconditionally from a selected address, so no per-column lane masks
+// stay live while the loads are in flight.
 template <int NB>
 __device__ __forceinline__ void load_lower(double (&a)[NB], const double *A, int lda, int n, int r)
 {
+    const double *row = (r < n) ? A + r : kZeroCell;
+    const size_t step = (r < n) ? (size_t)lda : 0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) a[c] = (c <= r && r < n) ? A[r + (size_t)c * lda] : 0.0;
+    for (int c = 0; c < NB; ++c) {
+        const double *p = (c <= r) ? row + (size_t)c * step : kZeroCell;
+        a[c] = *p;
+    }
 }
 
 template <int NB>
 __device__ __forceinline__ void store_lower(const double (&a)[NB], double *A, int lda, int n, int r)
 {
+    // A copy of r the compiler cannot see through, so that the 64-bit lane
+    // masks of load_lower are not kept alive until here.
+    int rr = r;
+    asm volatile("" : "+v"(rr));
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-        if (c <= r && r < n) A[r + (size_t)c * lda] = a[c];
+        if (c <= rr && rr < n) A[rr + (size_t)c * lda] = a[c];
+}
+
+// Value of v in lane src (src uniform across the wavefront).
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Right-looking factorisation in registers; returns 0 or 1 + first failing column.
@@ -69,26 +92,58 @@ __device__ __forceinline__ void store_lower(const double (&a)[NB], double *A, in
 // only ever update padding columns, so their NaNs never reach stored values).
 // The failing column is found afterwards: pivot k failed iff its square root
 // is not > 0 (d <= 0 gives NaN or +-0, NaN gives NaN), and nalgebra stops at
-// the first such k.
+// the first such k.  mydiag receives L_rr (the square root taken at step r).
 template <int NB, int G, bool EXACT>
-__device__ __forceinline__ int factor(double (&a)[NB], Smem<64 / G> &s, int n, int g, int r, int lane)
+__device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch, int n, int g, int r, int lane,
+                                      double &mydiag)
 {
-    double mydiag = 1.0;
+    mydiag = 1.0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        s.col[lane] = a[k];
-        __syncthreads();
-        const double den = sqrt(s.col[g * G + k]);
+        double dkk;
+        if constexpr (G == 64) {
+            dkk = lane_value(a[k], k);
+        } else {
+            col[lane] = a[k];
+            __syncthreads();
+            dkk = col[g * G + k];
+        }
+        const double den = sqrt(dkk);
         if (r == k) {
             a[k] = den;
             mydiag = den;
         } else if (r > k) {
             a[k] = a[k] / den;
         }
-        s.xch[0][lane] = a[k];
+        xch[lane] = a[k];
         __syncthreads();
+        // The multipliers are read in chunks of UC, one chunk ahead of its
+        // updates; the scheduling barriers stop the compiler from hoisting
+        // every read of the step (which would need 2*(NB-k) more VGPRs).
+        constexpr int UC = 8;
+        constexpr int NC = (NB + UC - 1) / UC;
+        const double *xg = xch + g * G;
+        const int cfirst = (k + 1) / UC;
+        double cur[UC];
 #pragma unroll
-        for (int j = k + 1; j < NB; ++j) a[j] = sub_prod<EXACT>(a[j], s.xch[0][g * G + j], a[k]);
+        for (int t = 0; t < UC; ++t) cur[t] = (cfirst * UC + t < NB) ? xg[cfirst * UC + t] : 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (c >= cfirst) {
+                double nxt[UC];
+#pragma unroll
+                for (int t = 0; t < UC; ++t)
+                    nxt[t] = (c + 1 < NC && (c + 1) * UC + t < NB) ? xg[(c + 1) * UC + t] : 0.0;
+#pragma unroll
+                for (int t = 0; t < UC; ++t) {
+                    const int j = c * UC + t;
+                    if (j > k && j < NB) a[j] = sub_prod<EXACT>(a[j], cur[t], a[k]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = 0; t < UC; ++t) cur[t] = nxt[t];
+            }
+        }
     }
     const bool bad = r < n && !(mydiag > 0.0);
     unsigned long long m = __ballot(bad);
@@ -236,6 +291,217 @@ __device__ __forceinline__ void backward(const double *L, int lda, double (&b)[C
     __syncthreads();
 }
 
+// ------------------------------------------------------------ fused factor + solve
+
+// Exact division with a reciprocal reused across numerators.  hipcc expands an
+// f64 x / d as s = div_scale(d), y = rcp(s) refined by two Newton steps,
+// q0 = x' * y, r = fma(-s, q0, x'), q = div_fmas(r, y, q0), div_fixup(q, d, x).
+// When 2^-100 <= |d| <= 2^100 and 2^-900 <= |x| <= 2^600 both scalings are
+// identities, div_fmas is a plain fma and the fixup returns q unchanged, so
+// div_with(x, d, recip_refined(d)) is bit-identical to x / d.  Outside that
+// range (zeros, infinities, NaNs, extreme exponents) callers use x / d.
+__device__ __forceinline__ double recip_refined(double d)
+{
+    const double r0 = __builtin_amdgcn_rcp(d);
+    const double e0 = __builtin_fma(-d, r0, 1.0);
+    const double r1 = __builtin_fma(r0, e0, r0);
+    const double e1 = __builtin_fma(-d, r1, 1.0);
+    return __builtin_fma(r1, e1, r1);
+}
+
+__device__ __forceinline__ double div_with(double x, double d, double y)
+{
+    const double q0 = x * y;
+    const double rem = __builtin_fma(-d, q0, x);
+    return __builtin_fma(rem, y, q0);
+}
+
+__device__ __forceinline__ bool div_den_ok(double d)
+{
+    const double ad = __builtin_fabs(d);
+    return ad >= 0x1p-100 && ad <= 0x1p100;
+}
+
+__device__ __forceinline__ bool div_num_ok(double x)
+{
+    const double ax = __builtin_fabs(x);
+    return ax >= 0x1p-900 && ax <= 0x1p600;
+}
+
+// a[i] for a wavefront-uniform i (a scalar branch tree, no scratch).
+template <int NB>
+__device__ __forceinline__ double reg_at(const double (&a)[NB], int i)
+{
+    double v = 0.0;
+    switch (i) {
+#define NROCM_REG_CASE(k) \
+    case k:               \
+        if constexpr (k < NB) v = a[k]; \
+        break;
+        NROCM_REG_CASE(0) NROCM_REG_CASE(1) NROCM_REG_CASE(2) NROCM_REG_CASE(3)
+        NROCM_REG_CASE(4) NROCM_REG_CASE(5) NROCM_REG_CASE(6) NROCM_REG_CASE(7)
+        NROCM_REG_CASE(8) NROCM_REG_CASE(9) NROCM_REG_CASE(10) NROCM_REG_CASE(11)
+        NROCM_REG_CASE(12) NROCM_REG_CASE(13) NROCM_REG_CASE(14) NROCM_REG_CASE(15)
+        NROCM_REG_CASE(16) NROCM_REG_CASE(17) NROCM_REG_CASE(18) NROCM_REG_CASE(19)
+        NROCM_REG_CASE(20) NROCM_REG_CASE(21) NROCM_REG_CASE(22) NROCM_REG_CASE(23)
+        NROCM_REG_CASE(24) NROCM_REG_CASE(25) NROCM_REG_CASE(26) NROCM_REG_CASE(27)
+        NROCM_REG_CASE(28) NROCM_REG_CASE(29) NROCM_REG_CASE(30) NROCM_REG_CASE(31)
+        NROCM_REG_CASE(32) NROCM_REG_CASE(33) NROCM_REG_CASE(34) NROCM_REG_CASE(35)
+        NROCM_REG_CASE(36) NROCM_REG_CASE(37) NROCM_REG_CASE(38) NROCM_REG_CASE(39)
+        NROCM_REG_CASE(40) NROCM_REG_CASE(41) NROCM_REG_CASE(42) NROCM_REG_CASE(43)
+        NROCM_REG_CASE(44) NROCM_REG_CASE(45) NROCM_REG_CASE(46) NROCM_REG_CASE(47)
+        NROCM_REG_CASE(48) NROCM_REG_CASE(49) NROCM_REG_CASE(50) NROCM_REG_CASE(51)
+        NROCM_REG_CASE(52) NROCM_REG_CASE(53) NROCM_REG_CASE(54) NROCM_REG_CASE(55)
+        NROCM_REG_CASE(56) NROCM_REG_CASE(57) NROCM_REG_CASE(58) NROCM_REG_CASE(59)
+        NROCM_REG_CASE(60) NROCM_REG_CASE(61) NROCM_REG_CASE(62) NROCM_REG_CASE(63)
+#undef NROCM_REG_CASE
+    }
+    return v;
+}
+
+template <int MPW>
+struct FusedSmem {
+    double col[64];
+    double xch[64];
+    double dg[64];            // L_rr per lane
+    double inv[64];           // recip_refined(L_rr) (EXACT) or 1 / L_rr (FAST)
+    double p[64 * CH];        // backward products, [lane][q]
+    double acc[MPW][8 * CH];  // dotx accumulators, [q][t]
+    double bi[MPW][CH];       // b_i of the current step
+    double x[MPW][CH];        // result of the current step
+};
+
+// Forward substitution L y = b (solve.rs:488-519) with L in registers:
+// coeff = b_i / L_ii; b_i = coeff; b_r = fl(fl(-coeff * L_ri) + b_r) for r > i.
+// Lane i computes the CH quotients and they are broadcast to its group
+// (readlane for one group per wavefront, LDS otherwise).
+template <int NB, int G, bool EXACT>
+__device__ __forceinline__ void forward_reg(const double (&a)[NB], double (&b)[CH], FusedSmem<64 / G> &s, int n,
+                                            int nmax, bool care, int qn, int g, int r, int lane, double mydiag,
+                                            double myinv, bool mydok)
+{
+    for (int i = 0; i < nmax; ++i) {
+        const double li = reg_at<NB>(a, i);
+        double t[CH];
+        // only lane i of each group that still solves decides (ok elsewhere)
+        const bool rel = care && i < n && r == i;
+        bool ok = !rel || mydok;
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            if constexpr (EXACT) {
+                t[q] = div_with(b[q], mydiag, myinv);
+                ok = ok && (!rel || q >= qn || div_num_ok(b[q]));
+            } else {
+                t[q] = b[q] * myinv;
+            }
+        }
+        if constexpr (EXACT) {
+            if (__ballot(!ok) != 0) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) t[q] = b[q] / mydiag;
+            }
+        }
+        double y[CH];
+        if constexpr (G == 64) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) y[q] = lane_value(t[q], i);
+        } else {
+            if (r == i) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) s.x[g][q] = t[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < CH; ++q) y[q] = s.x[g][q];
+            __syncthreads();
+        }
+        if (r == i) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) b[q] = y[q];
+        } else if (r > i) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], y[q], li);
+        }
+    }
+}
+
+// Backward substitution L^T x = y (solve.rs:697-755) with L in registers:
+// b_i = (b_i - dotc(L[i+1..,i], b[i+1..])) / L_ii, the dot in dotx's order
+// (acc_t sums positions t, t+8, ... of the full blocks of 8, then
+// 0 + (acc0+acc4) + (acc1+acc5) + (acc2+acc6) + (acc3+acc7), then the tail).
+// Products go through LDS; 8*CH lanes per group build the accumulators and CH
+// lanes finish the CH dots and quotients.
+template <int NB, int G, bool EXACT>
+__device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[CH], FusedSmem<64 / G> &s, int n,
+                                             int nmax, bool care, int qn, int g, int r, int lane)
+{
+    for (int i = nmax - 1; i >= 0; --i) {
+        const double li = reg_at<NB>(a, i);
+#pragma unroll
+        for (int q = 0; q < CH; ++q) s.p[lane * CH + q] = li * b[q];
+        if (r == i) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) s.bi[g][q] = b[q];
+        }
+        __syncthreads();
+        const int len = n - i - 1;
+        const int full = len > 0 ? (len & ~7) : 0;
+        const double *pg = s.p + (g * G + i + 1) * CH;
+        // Out-of-range slots add +0.0, an exact identity here: the running sums
+        // start at +0 and cannot become -0.
+#pragma unroll
+        for (int idx0 = 0; idx0 < 8 * CH; idx0 += G) {
+            const int idx = idx0 + r;
+            if (idx < 8 * CH) {
+                const int q = idx >> 3, t = idx & 7;
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int e = t + 8 * m;
+                    acc += (e < full) ? pg[e * CH + q] : 0.0;
+                }
+                s.acc[g][idx] = acc;
+            }
+        }
+        __syncthreads();
+        if (r < CH) {
+            const int q = r;
+            double res = 0.0;
+            if (len > 0) {
+                const double *ac = s.acc[g] + q * 8;
+                res += ac[0] + ac[4];
+                res += ac[1] + ac[5];
+                res += ac[2] + ac[6];
+                res += ac[3] + ac[7];
+#pragma unroll
+                for (int m = 0; m < 7; ++m) {
+                    const int e = full + m;
+                    res += (e < len) ? pg[e * CH + q] : 0.0;
+                }
+            }
+            const double num = s.bi[g][q] - res;
+            const double di = s.dg[g * G + i];
+            double xq;
+            if constexpr (EXACT) {
+                xq = div_with(num, di, s.inv[g * G + i]);
+                const bool ok = !care || i >= n || q >= qn || (div_num_ok(num) && div_den_ok(di));
+                if (__ballot(!ok) != 0) {
+                    if (!ok) xq = num / di;
+                }
+            } else {
+                xq = num * s.inv[g * G + i];
+            }
+            s.x[g][q] = xq;
+        }
+        __syncthreads();
+        if (r == i) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) b[q] = s.x[g][q];
+        }
+        __syncthreads();
+    }
+}
+
 // Member lookup shared by the kernels: order[begin..end) (sorted by decreasing
 // n), or with order == nullptr the single matrix (A1, B1, n1, lda1, ldb1).
 struct Member {
@@ -289,7 +555,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu
     double a[NB];
     load_lower<NB>(a, mb.A, mb.lda, mb.n, r);
     (void)nmax;
-    const int fail = factor<NB, G, EXACT>(a, s, mb.n, g, r, lane);
+    double mydiag;
+    const int fail = factor<NB, G, EXACT>(a, s.col, s.xch[0], mb.n, g, r, lane, mydiag);
     store_lower<NB>(a, mb.A, mb.lda, mb.n, r);
     if (mb.valid && r == 0 && info) info[order ? mb.jm : 0] = fail;
 }
@@ -323,6 +590,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 #pragma unroll
             for (int q = 0; q < CH; ++q)
                 if (c0 + q < nrhs) mb.B[r + (size_t)(c0 + q) * mb.ldb] = b[q];
+        }
+    }
+}
+
+// Factorisation and solve in one pass: the factored rows stay in registers
+// for the substitutions, so L is written once and never read back.  Members
+// whose factorisation failed keep their right-hand sides.
+template <int NB, int G, bool EXACT>
+__global__ __launch_bounds__(64) void potrf_potrs_kernel(const int *__restrict__ order, int begin, int end,
+                                                         const int *__restrict__ nsz,
+                                                         double *const *__restrict__ Aptr,
+                                                         double *const *__restrict__ Bptr, int nrhs,
+                                                         int *__restrict__ info)
+{
+    __shared__ FusedSmem<64 / G> s;
+    const int lane = threadIdx.x;
+    const int g = lane / G, r = lane % G;
+    const Member mb = member<G>(order, begin, end, nsz, Aptr, Bptr, nullptr, nullptr, 0, 0, 0, g);
+    const int n = mb.n;
+    const int nmax = __builtin_amdgcn_readfirstlane(n);
+    double a[NB];
+    load_lower<NB>(a, mb.A, mb.lda, n, r);
+    double mydiag;
+    const int fail = factor<NB, G, EXACT>(a, s.col, s.xch, n, g, r, lane, mydiag);
+    store_lower<NB>(a, mb.A, mb.lda, n, r);
+    if (mb.valid && r == 0) info[mb.jm] = fail;
+    if (nrhs <= 0 || Bptr == nullptr) return;
+
+    const double myinv = EXACT ? recip_refined(mydiag) : 1.0 / mydiag;
+    const bool mydok = div_den_ok(mydiag);
+    s.dg[lane] = mydiag;
+    s.inv[lane] = myinv;
+    __syncthreads();
+    const bool care = mb.valid && fail == 0;
+    for (int c0 = 0; c0 < nrhs; c0 += CH) {
+        const int qn = nrhs - c0 < CH ? nrhs - c0 : CH;
+        double b[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            b[q] = (care && r < n && q < qn) ? mb.B[r + (size_t)(c0 + q) * mb.ldb] : 0.0;
+        forward_reg<NB, G, EXACT>(a, b, s, n, nmax, care, qn, g, r, lane, mydiag, myinv, mydok);
+        backward_reg<NB, G, EXACT>(a, b, s, n, nmax, care, qn, g, r, lane);
+        if (care && r < n) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+                if (q < qn) mb.B[r + (size_t)(c0 + q) * mb.ldb] = b[q];
         }
     }
 }
@@ -410,7 +723,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     int nrocm_chol_bwd_launch_##NB(unsigned grid, hipStream_t st, const int *order, int begin,    \
                                    int end, const int *nsz, const double *const *L,               \
                                    const double *const *X, const double *const *dX,               \
-                                   double *const *dA, double *const *dB, int nrhs);
+                                   double *const *dA, double *const *dB, int nrhs);               \
+    int nrocm_chol_fused_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     double *const *B, int nrhs, int *info);
 
 #define NROCM_CHOL_DEFINE(NB)                                                                     \
     int nrocm_chol_potrf_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
@@ -451,6 +767,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         constexpr int G = group_for<NB>();                                                        \
         hipLaunchKernelGGL((potrs_backward_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st,     \
                            order, begin, end, nsz, L, X, dX, dA, dB, nrhs);                       \
+        return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
+    }                                                                                             \
+    int nrocm_chol_fused_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+                                     int begin, int end, const int *nsz, double *const *A,        \
+                                     double *const *B, int nrhs, int *info)                       \
+    {                                                                                             \
+        using namespace nrocm_chol;                                                               \
+        constexpr int G = group_for<NB>();                                                        \
+        if (exact)                                                                                \
+            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st,    \
+                               order, begin, end, nsz, A, B, nrhs, info);                         \
+        else                                                                                      \
+            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st,   \
+                               order, begin, end, nsz, A, B, nrhs, info);                         \
         return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
     }
 

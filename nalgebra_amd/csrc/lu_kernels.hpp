@@ -256,54 +256,62 @@ template <typename T>
 __global__ __launch_bounds__(256) void panel_cols_kernel(T *A, int lda, int m, int b0, int w, int cl, int cr0,
                                                          int cr1, const int *ipiv, int exact)
 {
-    __shared__ int src[64];      // touched row positions
-    __shared__ T val[64];
+    __shared__ int pos[64];      // touched row positions
+    __shared__ int who[64];      // original position whose value ends at pos[q]
+    __shared__ int piv[32];
     __shared__ int nt;
-    __shared__ T U[64];
+    __shared__ T L11[32][33];
+    __shared__ T U[32];
     const int t = threadIdx.x;
     const int idx = blockIdx.x;
     // columns [cl, b0) swap only; columns [cr0, cr1) swap + solve + update
     const int c = idx < b0 - cl ? cl + idx : cr0 + (idx - (b0 - cl));
     const bool full = c >= cr0;
     T *col = A + (size_t)c * lda;
-    // ---- compose the w swaps on the touched rows (positions b0..b0+w-1 and their partners)
+    if (t < w) piv[t] = ipiv[b0 + t];
+    if (full)
+        for (int e = t; e < w * w; e += blockDim.x)
+            L11[e % w][e / w] = A[(b0 + e % w) + (size_t)(b0 + e / w) * lda];
+    __syncthreads();
+    // compose the w swaps on indices only
     if (t == 0) {
         int n_ = 0;
         for (int i = 0; i < w; ++i) {
-            const int p = ipiv[b0 + i];
+            const int p = piv[i];
             int a = -1, b = -1;
             for (int q = 0; q < n_; ++q) {
-                if (src[q] == b0 + i) a = q;
-                if (src[q] == p) b = q;
+                if (pos[q] == b0 + i) a = q;
+                if (pos[q] == p) b = q;
             }
-            if (a < 0) { a = n_; src[n_] = b0 + i; val[n_] = col[b0 + i]; ++n_; }
-            if (b < 0) { b = n_; src[n_] = p; val[n_] = col[p]; ++n_; }
-            if (a != b) {
-                const T tv = val[a];
-                val[a] = val[b];
-                val[b] = tv;
-            }
+            if (a < 0) { a = n_; pos[n_] = b0 + i; who[n_] = b0 + i; ++n_; }
+            if (b < 0) { b = n_; pos[n_] = p; who[n_] = p; ++n_; }
+            const int tw = who[a];
+            who[a] = who[b];
+            who[b] = tw;
         }
         nt = n_;
     }
     __syncthreads();
-    if (t < nt) col[src[t]] = val[t];
+    T v = T(0);
+    if (t < nt) v = col[who[t]];
+    __syncthreads();
+    if (t < nt) col[pos[t]] = v;
     __syncthreads();
     if (!full) return;
-    // ---- U12 for this column: rows b0..b0+w, unit lower L11 (one wavefront)
+    // U12 for this column: rows b0..b0+w, unit lower L11 (one wavefront)
     if (t < 64) {
-        T v = t < w ? col[b0 + t] : T(0);
+        T u = t < w ? col[b0 + t] : T(0);
         for (int i = 0; i < w - 1; ++i) {
-            const T u = __shfl(v, i);
-            if (t > i && t < w) v = sub_prod(v, u, A[(b0 + t) + (size_t)(b0 + i) * lda], exact != 0);
+            const T ui = __shfl(u, i);
+            if (t > i && t < w) u = sub_prod(u, ui, L11[t][i], exact != 0);
         }
         if (t < w) {
-            col[b0 + t] = v;
-            U[t] = v;
+            col[b0 + t] = u;
+            U[t] = u;
         }
     }
     __syncthreads();
-    // ---- rank-w update of rows b0+w..m
+    // rank-w update of rows b0+w..m, terms in increasing k
     for (int r = b0 + w + t; r < m; r += blockDim.x) {
         T a = col[r];
         for (int q = 0; q < w; ++q) a = sub_prod(a, U[q], A[r + (size_t)(b0 + q) * lda], exact != 0);
