@@ -359,13 +359,15 @@ __device__ __forceinline__ double reg_at(const double (&a)[NB], int i)
     return v;
 }
 
+constexpr int PSTR = 64 + 4;
+
 template <int MPW>
 struct FusedSmem {
     double col[64];
     double xch[64];
     double dg[64];            // L_rr per lane
     double inv[64];           // recip_refined(L_rr) (EXACT) or 1 / L_rr (FAST)
-    double p[64 * CH];        // backward products, [lane][q]
+    double p[CH][PSTR];       // backward products, [q][lane] (padded rows: no bank conflicts)
     double acc[MPW][8 * CH];  // dotx accumulators, [q][t]
     double bi[MPW][CH];       // b_i of the current step
     double x[MPW][CH];        // result of the current step
@@ -438,7 +440,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
     for (int i = nmax - 1; i >= 0; --i) {
         const double li = reg_at<NB>(a, i);
 #pragma unroll
-        for (int q = 0; q < CH; ++q) s.p[lane * CH + q] = li * b[q];
+        for (int q = 0; q < CH; ++q) s.p[q][lane] = li * b[q];
         if (r == i) {
 #pragma unroll
             for (int q = 0; q < CH; ++q) s.bi[g][q] = b[q];
@@ -446,7 +448,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
         __syncthreads();
         const int len = n - i - 1;
         const int full = len > 0 ? (len & ~7) : 0;
-        const double *pg = s.p + (g * G + i + 1) * CH;
+        const int pg = g * G + i + 1;  // first row of the dot in s.p
         // Out-of-range slots add +0.0, an exact identity here: the running sums
         // start at +0 and cannot become -0.
 #pragma unroll
@@ -458,7 +460,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const int e = t + 8 * m;
-                    acc += (e < full) ? pg[e * CH + q] : 0.0;
+                    acc += (e < full) ? s.p[q][pg + e] : 0.0;
                 }
                 s.acc[g][idx] = acc;
             }
@@ -476,7 +478,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
 #pragma unroll
                 for (int m = 0; m < 7; ++m) {
                     const int e = full + m;
-                    res += (e < len) ? pg[e * CH + q] : 0.0;
+                    res += (e < len) ? s.p[q][pg + e] : 0.0;
                 }
             }
             const double num = s.bi[g][q] - res;

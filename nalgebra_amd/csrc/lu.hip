@@ -26,26 +26,20 @@ struct PanelCfg {
     int rpt, w;
 };
 
-// Register budget of the one-workgroup panel: RPT * W values per thread.
+// Rows per thread and width of the one-workgroup panel for `rows` rows.
 template <typename T>
 PanelCfg<T> panel_cfg(int rows)
 {
-    constexpr int budget = sizeof(T) == 8 ? 32 : 64;
     int rpt = 1;
     while (rpt * PANEL_THREADS < rows) rpt *= 2;
-    int w = budget / rpt;
-    if (w > 32) w = 32;
-    if (w < 1) w = 1;
-    return {rpt, w};
+    return {rpt, panel_width(rpt, (int)sizeof(T))};
 }
 
 template <typename T, int RPT>
 hipError_t launch_panel_rpt(hipStream_t st, int w_cap, T *A, int lda, int m, int col0, int w, int *ipiv,
                             int *info)
 {
-    constexpr int budget = sizeof(T) == 8 ? 32 : 64;
-    constexpr int WMAX0 = budget / RPT;
-    constexpr int WMAX = WMAX0 > 32 ? 32 : (WMAX0 < 1 ? 1 : WMAX0);
+    constexpr int WMAX = panel_width(RPT, (int)sizeof(T));
     (void)w_cap;
     hipLaunchKernelGGL((panel_kernel<T, RPT, WMAX>), dim3(1), dim3(PANEL_THREADS), 0, st, A, lda, m, col0, w,
                        ipiv, info);
@@ -137,7 +131,7 @@ int getrf_impl(nrocm_handle h, hipStream_t st, int m, int n, T *A, int lda, int 
     LU_TRY(hipGetLastError());
     const int mn = std::min(m, n);
     if (mn == 0) return NROCM_OK;
-    if (m > 32 * PANEL_THREADS) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "getrf: more than 32768 rows");
+    if (m > 32 * PANEL_THREADS) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "getrf: more than 16384 rows");
     const bool exact = trail == TRAIL_EXACT;
     constexpr int NB = 64;
     for (int j0 = 0; j0 < mn; j0 += NB) {

@@ -15,7 +15,17 @@
 
 namespace nrocm_lu {
 
-constexpr int PANEL_THREADS = 1024;
+// Two wavefronts per SIMD (up to 256 VGPRs per thread); the per-column
+// barrier synchronises eight wavefronts.
+constexpr int PANEL_THREADS = 512;
+
+// Panel width per rows-per-thread: RPT * W values of the panel plus the W
+// values of the pivot row stay in registers without spilling.
+__host__ __device__ constexpr int panel_width(int rpt, int elem_bytes)
+{
+    return elem_bytes == 8 ? (rpt <= 1 ? 32 : rpt == 2 ? 16 : rpt == 4 ? 8 : rpt == 8 ? 4 : rpt == 16 ? 2 : 1)
+                           : (rpt <= 2 ? 32 : rpt == 4 ? 16 : rpt == 8 ? 8 : rpt == 16 ? 4 : 2);
+}
 
 // pivot candidate: |value| and global row; `first-index strict max` order.
 struct Cand {
@@ -31,6 +41,15 @@ __device__ __forceinline__ bool beats(double av, int ai, double bv, int bi)
     return av > bv || (av == bv && ai < bi);
 }
 
+// Returns v unchanged.  Copy loops whose values pass through this are not
+// turned into memcpy calls, which would keep register arrays in scratch.
+template <typename T>
+__device__ __forceinline__ T pass(T v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T sub_prod(T acc, T x, T y, bool exact)
 {
@@ -42,7 +61,7 @@ __device__ __forceinline__ T sub_prod(T acc, T x, T y, bool exact)
 }
 
 // Unblocked LU of the panel A[row0.., col0..col0+w) held in registers by one
-// 1024-thread workgroup; thread t holds rows row0 + t + 1024*s, s < RPT.
+// PANEL_THREADS-thread workgroup; thread t holds rows row0 + t + PANEL_THREADS*s, s < RPT.
 // Pivot rows are recorded as global row indices in ipiv[col0 + c].
 template <typename T, int RPT, int W>
 __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(T *A, int lda, int m, int col0, int w,
@@ -63,80 +82,88 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(T *A, int lda, int
         for (int c = 0; c < W; ++c) x[s][c] = (r < m && c < w) ? A[r + (size_t)(col0 + c) * lda] : T(0);
     }
 
+    // No break/continue in this loop: it must unroll completely so that x
+    // stays in registers.
 #pragma unroll
     for (int c = 0; c < W; ++c) {
-        if (c >= w) break;
-        const int i = row0 + c;  // pivot row position (global)
-        // ---- pivot search over rows i..m-1 of column c
-        double bv = -1.0;
-        int bi = 0x7fffffff;
+        if (c < w) {
+            const int i = row0 + c;  // pivot row position (global)
+            // ---- pivot search over rows i..m-1 of column c
+            double bv = -1.0;
+            int bi = 0x7fffffff;
 #pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-            const int r = row0 + t + PANEL_THREADS * s;
-            if (r >= i && r < m) {
-                double v = fabs((double)x[s][c]);
-                if (v != v) v = (r == i) ? __builtin_inf() : -1.0;  // NaN rule
-                if (beats(v, r, bv, bi)) { bv = v; bi = r; }
+            for (int s = 0; s < RPT; ++s) {
+                const int r = row0 + t + PANEL_THREADS * s;
+                if (r >= i && r < m) {
+                    double v = fabs((double)x[s][c]);
+                    if (v != v) v = (r == i) ? __builtin_inf() : -1.0;  // NaN rule
+                    if (beats(v, r, bv, bi)) { bv = v; bi = r; }
+                }
             }
-        }
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int oi = __shfl_xor(bi, off);
-            if (beats(ov, oi, bv, bi)) { bv = ov; bi = oi; }
-        }
-        if (lane == 0) { red_v[wv] = bv; red_i[wv] = bi; }
-        __syncthreads();
-        bv = red_v[0];
-        bi = red_i[0];
-#pragma unroll
-        for (int q = 1; q < PANEL_THREADS / 64; ++q)
-            if (beats(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
-        const int piv = bi;
-        // ---- exchange rows i and piv through LDS
-        T(*rb)[W] = rowbuf[c & 1];
-#pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-            const int r = row0 + t + PANEL_THREADS * s;
-            if (r == piv) {
-#pragma unroll
-                for (int q = 0; q < W; ++q) rb[0][q] = x[s][q];
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (beats(ov, oi, bv, bi)) { bv = ov; bi = oi; }
             }
-            if (r == i) {
+            if (lane == 0) { red_v[wv] = bv; red_i[wv] = bi; }
+            __syncthreads();
+            bv = red_v[0];
+            bi = red_i[0];
 #pragma unroll
-                for (int q = 0; q < W; ++q) rb[1][q] = x[s][q];
+            for (int q = 1; q < PANEL_THREADS / 64; ++q)
+                if (beats(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
+            const int piv = bi;
+            // ---- exchange rows i and piv through LDS
+            T(*rb)[W] = rowbuf[c & 1];
+#pragma unroll
+            for (int s = 0; s < RPT; ++s) {
+                const int r = row0 + t + PANEL_THREADS * s;
+                if (r == piv) {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) rb[0][q] = pass(x[s][q]);
+                }
+                if (r == i) {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) rb[1][q] = pass(x[s][q]);
+                }
             }
-        }
-        __syncthreads();
-        const T d = rb[0][c];  // the pivot value
-        if (t == 0) ipiv[col0 + c] = (d == T(0)) ? i : piv;
-        if (d == T(0)) {
-            // lu.rs:107-110: all-zero column, nothing is swapped or updated
-            if (t == 0 && info && *info == 0) *info = i + 1;
-            continue;
-        }
-        const T inv = T(1) / d;
+            __syncthreads();
+            const T d = rb[0][c];  // the pivot value
+            if (t == 0) ipiv[col0 + c] = (d == T(0)) ? i : piv;
+            if (d == T(0)) {
+                // lu.rs:107-110: all-zero column, nothing is swapped or updated
+                if (t == 0 && info && *info == 0) *info = i + 1;
+            } else {
+                const T inv = T(1) / d;
 #pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-            const int r = row0 + t + PANEL_THREADS * s;
-            if (r == i) {
+                for (int s = 0; s < RPT; ++s) {
+                    const int r = row0 + t + PANEL_THREADS * s;
+                    if (r == i) {
 #pragma unroll
-                for (int q = 0; q < W; ++q) x[s][q] = rb[0][q];
-            } else if (r == piv) {
+                        for (int q = 0; q < W; ++q) x[s][q] = pass(rb[0][q]);
+                    } else if (r == piv) {
 #pragma unroll
-                for (int q = 0; q < W; ++q) x[s][q] = rb[1][q];
-            }
-            if (r > i && r < m) {
-                const T l = x[s][c] * inv;
-                x[s][c] = l;
+                        for (int q = 0; q < W; ++q) x[s][q] = pass(rb[1][q]);
+                    }
+                    if (r > i && r < m) {
+                        const T l = x[s][c] * inv;
+                        x[s][c] = l;
 #pragma unroll
-                for (int q = c + 1; q < W; ++q) x[s][q] = sub_prod(x[s][q], rb[0][q], l, true);
+                        for (int q = 0; q < W; ++q)  // constant trip count: unrolls before c does
+                            if (q > c) x[s][q] = sub_prod(x[s][q], rb[0][q], l, true);
+                    }
+                }
             }
         }
     }
+    // A copy of t the compiler cannot see through, so the load addresses and
+    // row predicates are recomputed here instead of living in registers.
+    int tt = t;
+    asm volatile("" : "+v"(tt));
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
-        const int r = row0 + t + PANEL_THREADS * s;
+        const int r = row0 + tt + PANEL_THREADS * s;
         if (r < m) {
 #pragma unroll
             for (int c = 0; c < W; ++c)

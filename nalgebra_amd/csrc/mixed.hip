@@ -6,7 +6,8 @@
 //                                      factors a copy of A with FP64 MFMA
 //   x = 0, r = b
 //   repeat:  d = U^{-1} L^{-1} P r     (working precision, single-launch
-//                                       look-back triangular solves)
+//                                       look-back triangular solves with the
+//                                       64x64 diagonal blocks pre-inverted)
 //            x += d                    (FP64)
 //            r = b - A x               (FP64, deterministic two-pass GEMV)
 //            berr = |r|_inf / (|A|_inf |x|_inf + |b|_inf)
@@ -75,18 +76,63 @@ __global__ void zero_int_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
-// Single-launch blocked triangular solve of L y = v (unit lower, LOWER) or
-// U y = v (non-unit upper).  Work items are 64-row blocks, taken in solve order
-// through an atomic ticket, so every block a workgroup waits for is owned by a
-// workgroup that is already running.  Waits are bounded: on timeout the
-// error word is set and the kernel still exits.
+// Inverses of the 64x64 diagonal blocks of the packed L\U factor: the unit
+// lower blocks of L (LOWER) or the upper blocks of U.  One wavefront per block;
+// lane c computes column c of the inverse by substitution against the block
+// held in LDS (rows past n are padded with the identity).
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict__ M, int lda, int n, T *v, int *ticket,
+__global__ __launch_bounds__(64) void inv_diag_kernel(const T *__restrict__ M, int lda, int n, T *__restrict__ Inv)
+{
+    __shared__ T D[BS][BS + 1];
+    const int R = blockIdx.x;
+    const int r0 = R * BS;
+    const int rows = min(BS, n - r0);
+    const int c = threadIdx.x;
+    for (int e = c; e < BS * BS; e += 64) {
+        const int a = e % BS, b = e / BS;
+        D[a][b] = (a < rows && b < rows) ? M[(r0 + a) + (size_t)(r0 + b) * lda] : (a == b ? T(1) : T(0));
+    }
+    __syncthreads();
+    T x[BS];
+    if constexpr (LOWER) {
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            T sacc = (i == c) ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < BS; ++k)  // constant trip count: unrolls before i does
+                if (k < i) sacc = fma(-D[i][k], x[k], sacc);
+            x[i] = sacc;
+        }
+    } else {
+#pragma unroll
+        for (int i = BS - 1; i >= 0; --i) {
+            T sacc = (i == c) ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < BS; ++k)
+                if (k > i) sacc = fma(-D[i][k], x[k], sacc);
+            x[i] = sacc / D[i][i];
+        }
+    }
+    T *out = Inv + (size_t)R * BS * BS + (size_t)c * BS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i) out[i] = x[i];
+}
+
+// Single-launch blocked triangular solve of L y = v (unit lower, LOWER) or
+// U y = v (non-unit upper) with the inverted diagonal blocks.  Work items are
+// 64-row blocks, taken in solve order through an atomic ticket, so every block
+// a workgroup waits for is owned by a workgroup that is already running.  The
+// off-diagonal block of step j+1 is loaded while step j waits.  Waits are
+// bounded: on timeout the error word is set and the kernel still exits.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict__ M, int lda, int n,
+                                                            const T *__restrict__ Inv, T *v, int *ticket,
                                                             int *flags, int *err)
 {
+    constexpr int PER = BS / 4;  // columns per thread and block
     __shared__ T xs[BS];
     __shared__ T part[4][BS];
-    __shared__ T D[BS][BS + 1];
+    __shared__ T ys[BS];
     __shared__ int my;
     const int tid = threadIdx.x;
     const int nblk = (n + BS - 1) / BS;
@@ -97,11 +143,34 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
     const int r0 = R * BS;
     const int rows = min(BS, n - r0);
     const int rr = tid & 63, sl = tid >> 6;         // row within block, column slice
+    T inv[PER];
+    {
+        const T *IR = Inv + (size_t)R * BS * BS;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) inv[q] = IR[rr + (size_t)(sl + 4 * q) * BS];
+    }
+    auto load_block = [&](int j, T (&dst)[PER]) {
+        const int J = LOWER ? j : nblk - 1 - j;
+        const int c0 = J * BS;
+        const int cols = min(BS, n - c0);
+        const T *src = M + (r0 + rr) + (size_t)c0 * lda;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int c = sl + 4 * q;
+            dst[q] = (rr < rows && c < cols) ? src[(size_t)c * lda] : T(0);
+        }
+    };
     T acc = 0;
+    T nxt[PER];
+    if (k > 0) load_block(0, nxt);
     for (int j = 0; j < k; ++j) {
         const int J = LOWER ? j : nblk - 1 - j;
         const int c0 = J * BS;
         const int cols = min(BS, n - c0);
+        T cur[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) cur[q] = nxt[q];
+        if (j + 1 < k) load_block(j + 1, nxt);
         if (tid == 0) {
             long spins = 0;
             while (__hip_atomic_load(&flags[J], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -109,41 +178,29 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
                     atomicExch(err, 1);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         __syncthreads();
         if (tid < cols) xs[tid] = __hip_atomic_load(&v[c0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (rr < rows) {
-            const T *col = M + (r0 + rr) + (size_t)c0 * lda;
-            for (int c = sl; c < cols; c += 4) acc = fma(-col[(size_t)c * lda], xs[c], acc);
-        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) acc = fma(-cur[q], xs[sl + 4 * q], acc);  // cur is 0 outside the block
         __syncthreads();
     }
     part[sl][rr] = acc;
-    // diagonal block
-    for (int e = tid; e < BS * BS; e += 256) {
-        const int a = e % BS, b = e / BS;
-        D[a][b] = (a < rows && b < rows) ? M[(r0 + a) + (size_t)(r0 + b) * lda] : T(0);
-    }
     __syncthreads();
-    if (tid < 64) {
-        T y = 0;
-        if (tid < rows) y = v[r0 + tid] + ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
-        if (LOWER) {
-            for (int i = 0; i < rows; ++i) {
-                const T yi = __shfl(y, i);
-                if (tid > i && tid < rows) y = fma(-D[tid][i], yi, y);
-            }
-        } else {
-            for (int i = rows - 1; i >= 0; --i) {
-                if (tid == i) y = y / D[i][i];
-                const T yi = __shfl(y, i);
-                if (tid < i) y = fma(-D[tid][i], yi, y);
-            }
-        }
-        if (tid < rows) __hip_atomic_store(&v[r0 + tid], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < BS) ys[tid] = (tid < rows) ? v[r0 + tid] + ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
+                                         : T(0);
+    __syncthreads();
+    T sacc = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) sacc = fma(inv[q], ys[sl + 4 * q], sacc);
+    part[sl][rr] = sacc;
+    __syncthreads();
+    if (tid < rows) {
+        const T y = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+        __hip_atomic_store(&v[r0 + tid], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (tid == 0) {
@@ -235,19 +292,20 @@ __global__ void zero_d_kernel(double *p, int n)
     } while (0)
 
 template <typename T>
-int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, T *v, int *sync, int *err)
+int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const T *InvL, const T *InvU, T *v,
+              int *sync, int *err)
 {
     const int nblk = (n + BS - 1) / BS;
     int *ticket = sync, *flags = sync + 1;
     hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
     IR_TRY(hipGetLastError());
-    hipLaunchKernelGGL((trsv_lookback_kernel<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, n, v, ticket, flags,
-                       err);
+    hipLaunchKernelGGL((trsv_lookback_kernel<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvL, v, ticket,
+                       flags, err);
     IR_TRY(hipGetLastError());
     hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
     IR_TRY(hipGetLastError());
-    hipLaunchKernelGGL((trsv_lookback_kernel<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, n, v, ticket, flags,
-                       err);
+    hipLaunchKernelGGL((trsv_lookback_kernel<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvU, v, ticket,
+                       flags, err);
     IR_TRY(hipGetLastError());
     return NROCM_OK;
 }
@@ -266,7 +324,7 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     const size_t oW = take(szW), oIpiv = take(n * sizeof(int)), oPerm = take(n * sizeof(int)),
                  oR = take(szVec), oD = take((size_t)n * sizeof(T)), oPart = take(GEMV_CS * szVec),
                  oSync = take((nblk + 2) * sizeof(int)), oRed = take(8 * sizeof(unsigned long long)),
-                 oInfo = take(2 * sizeof(int));
+                 oInfo = take(2 * sizeof(int)), oInv = take(2 * (size_t)nblk * BS * BS * sizeof(T));
     void *ws = nullptr;
     int s = nrocm_workspace(h, off, &ws);
     if (s != NROCM_OK) return s;
@@ -278,6 +336,7 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     int *sync = (int *)(base + oSync);
     unsigned long long *red = (unsigned long long *)(base + oRed);  // [0]=|A|, [1]=|b|, [2]=|r|, [3]=|x|
     int *info = (int *)(base + oInfo), *err = info + 1;
+    T *InvL = (T *)(base + oInv), *InvU = InvL + (size_t)nblk * BS * BS;
 
     const size_t tot = (size_t)n * n;
     if constexpr (sizeof(T) == 4)
@@ -294,6 +353,10 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
         fs = nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
     if (fs != NROCM_OK) return fs;
     hipLaunchKernelGGL(perm_kernel, dim3(1), dim3(1024), n * sizeof(int), st, ipiv, n, perm);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL((inv_diag_kernel<T, true>), dim3(nblk), dim3(64), 0, st, W, n, n, InvL);
+    IR_TRY(hipGetLastError());
+    hipLaunchKernelGGL((inv_diag_kernel<T, false>), dim3(nblk), dim3(64), 0, st, W, n, n, InvU);
     IR_TRY(hipGetLastError());
     // norms of A (row sums) and b
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red, 8);
@@ -312,7 +375,7 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     for (it = 1; it <= maxit; ++it) {
         hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
         IR_TRY(hipGetLastError());
-        s = trsv_pair<T>(h, st, W, n, n, d, sync, err);
+        s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
         if (s != NROCM_OK) return s;
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
         hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
