@@ -7,7 +7,8 @@
 //       panel_kernel        one workgroup factors the block's columns (rows b0..m)
 //       laswp               the block's swaps on the other panel columns
 //       trsm + update       U12 and the rank-w update of the rest of the panel
-//     laswp                 the panel's swaps on all columns outside it
+//     swap plan + apply     the panel's swaps, composed into one row plan,
+//                           on all columns outside it
 //     trsm                  U12 = L11^{-1} A12 for the trailing columns
 //     trailing update       A22 -= L21 * U12: EXACT = ordered unfused SIMT,
 //                           FAST = FP64 MFMA, float working copies: FP32 or BF16 MFMA
@@ -151,8 +152,16 @@ int getrf_impl(nrocm_handle h, hipStream_t st, int m, int n, T *A, int lda, int 
             }
             b0 += w;
         }
-        LU_TRY(laswp<T>(st, A, lda, 0, j0, ipiv, j0, j1));
-        LU_TRY(laswp<T>(st, A, lda, j1, n, ipiv, j0, j1));
+        // the panel's swaps on every column outside it, through a composed row plan
+        if (j0 > 0 || j1 < n) {
+            int *plan = (int *)h->small + NROCM_SMALL_PLAN;
+            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(PLAN_MAX), 0, st, ipiv, j0, j1, plan);
+            LU_TRY(hipGetLastError());
+            const int ncol = j0 + (n - j1);
+            hipLaunchKernelGGL((apply_plan_kernel<T>), dim3(nrocm_cdiv(ncol, 2)), dim3(256), 0, st, A, lda, plan, 0,
+                               j0, j1, n);
+            LU_TRY(hipGetLastError());
+        }
         if (j1 < n) {
             LU_TRY(trsm<T>(st, A, lda, j0, nb, j1, n, exact));
             LU_TRY(trailing<T>(st, trail, A, lda, j1, m, j1, n, j0, nb));

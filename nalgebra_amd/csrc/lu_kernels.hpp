@@ -338,12 +338,109 @@ __global__ __launch_bounds__(256) void panel_cols_kernel(T *A, int lda, int m, i
         }
     }
     __syncthreads();
-    // rank-w update of rows b0+w..m, terms in increasing k
-    for (int r = b0 + w + t; r < m; r += blockDim.x) {
-        T a = col[r];
-        for (int q = 0; q < w; ++q) a = sub_prod(a, U[q], A[r + (size_t)(b0 + q) * lda], exact != 0);
-        col[r] = a;
+    // rank-w update of rows b0+w..m, terms in increasing k; RU rows per
+    // thread have their loads in flight together
+    constexpr int RU = 8;
+    for (int rb = b0 + w + t; rb < m; rb += 256 * RU) {
+        T a[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int r = rb + 256 * u;
+            a[u] = r < m ? col[r] : T(0);
+        }
+        for (int q = 0; q < w; ++q) {
+            const T uq = U[q];
+            const T *Lq = A + (size_t)(b0 + q) * lda;
+            T l[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+                const int r = rb + 256 * u;
+                l[u] = r < m ? Lq[r] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < RU; ++u) a[u] = sub_prod(a[u], uq, l[u], exact != 0);
+        }
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int r = rb + 256 * u;
+            if (r < m) col[r] = a[u];
+        }
     }
+}
+
+// Composition of the row swaps ipiv[k0..k1) (k1 - k0 <= 64, applied in
+// order, as LU::new swaps whole rows at every step) into a row plan:
+// plan[0] = number nt of touched positions; after the swaps, position
+// plan[1 + s] holds the value that was at position plan[1 + PLAN_MAX + s].
+constexpr int PLAN_MAX = 128;
+
+__global__ __launch_bounds__(128) void swap_plan_kernel(const int *ipiv, int k0, int k1, int *plan)
+{
+    __shared__ int pos[PLAN_MAX], who[PLAN_MAX], pv[PLAN_MAX / 2];
+    __shared__ int cnt, sa, sb;
+    const int t = threadIdx.x;
+    if (t < k1 - k0) pv[t] = ipiv[k0 + t];
+    if (t == 0) cnt = 0;
+    __syncthreads();
+    for (int i = k0; i < k1; ++i) {
+        const int p = pv[i - k0];
+        if (p == i) continue;  // uniform: no swap at this step
+        if (t == 0) {
+            sa = -1;
+            sb = -1;
+        }
+        __syncthreads();
+        if (t < cnt) {
+            const int ps = pos[t];  // positions are distinct: at most one match each
+            if (ps == i) sa = t;
+            if (ps == p) sb = t;
+        }
+        __syncthreads();
+        if (t == 0) {
+            int a = sa, b = sb;
+            if (a < 0) {
+                a = cnt++;
+                pos[a] = i;
+                who[a] = i;
+            }
+            if (b < 0) {
+                b = cnt++;
+                pos[b] = p;
+                who[b] = p;
+            }
+            const int tw = who[a];
+            who[a] = who[b];
+            who[b] = tw;
+        }
+        __syncthreads();
+    }
+    const int nt = cnt;
+    if (t == 0) plan[0] = nt;
+    if (t < nt) {
+        plan[1 + t] = pos[t];
+        plan[1 + PLAN_MAX + t] = who[t];
+    }
+}
+
+// Applies a row plan to the columns [ca0, ca1) and [cb0, cb1): two columns
+// per workgroup, one thread per touched position; every value is read before
+// any is written.
+template <typename T>
+__global__ __launch_bounds__(256) void apply_plan_kernel(T *A, int lda, const int *plan, int ca0, int ca1, int cb0,
+                                                         int cb1)
+{
+    const int t = threadIdx.x;
+    const int slot = t & (PLAN_MAX - 1);
+    const int na = ca1 > ca0 ? ca1 - ca0 : 0;
+    const int nbc = cb1 > cb0 ? cb1 - cb0 : 0;
+    const int ci = blockIdx.x * 2 + (t >> 7);
+    const bool valid = ci < na + nbc;
+    const int c = ci < na ? ca0 + ci : cb0 + (ci - na);
+    const int nt = plan[0];
+    T v = T(0);
+    if (valid && slot < nt) v = A[plan[1 + PLAN_MAX + slot] + (size_t)c * lda];
+    __syncthreads();
+    if (valid && slot < nt) A[plan[1 + slot] + (size_t)c * lda] = v;
 }
 
 __global__ void fill_int_kernel(int *p, int n, int v)

@@ -10,6 +10,8 @@
 
 #include "../../include/nalgebra_rocm.h"
 
+constexpr int NROCM_SMALL_PLAN = 256;  // int offset of the LU row plan (257 ints) in small
+
 struct nrocm_handle_s {
     hipStream_t stream = nullptr;   // caller's stream (nullptr = legacy default stream)
     int device = 0;
@@ -20,7 +22,7 @@ struct nrocm_handle_s {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     // Small device-side scalars (counters, flags, norms).
-    void *small = nullptr;          // 4 KiB
+    void *small = nullptr;          // 4 KiB: int[0] Cholesky block info, ints from NROCM_SMALL_PLAN: LU row plan
 
     // hipGraph cache for long launch sequences (keyed by every argument that
     // shapes the sequence), captured on a private stream, replayed on `stream`.
