@@ -9,10 +9,6 @@ namespace nrocm_chol {
 
 constexpr int CH = 4;  // right-hand sides held in registers at a time
 
-// Occupancy target per padded size: a[NB] doubles live in registers.
-template <int NB>
-constexpr int waves_per_eu() { return NB <= 16 ? 8 : NB <= 32 ? 5 : NB <= 48 ? 4 : 3; }
-
 template <int NB>
 constexpr int group_for() { return NB <= 8 ? 8 : NB <= 16 ? 16 : NB <= 32 ? 32 : 64; }
 
@@ -23,13 +19,15 @@ __device__ __forceinline__ double sub_prod(double acc, double x, double y)
     // One asm block per update keeps the product next to its subtraction: as
     // separate IR operations the scheduler hoists every product of a step and
     // spills the row array to scratch.
+    double r;
     if constexpr (EXACT) {
-        double r;
         asm("v_mul_f64 %0, %1, %2\n\tv_add_f64 %0, %3, -%0" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
-        return r;
     } else {
-        return __builtin_fma(-x, y, acc);
+        // as an asm statement for the same reason: a plain fma lets the
+        // scheduler raise register use to 256 VGPRs
+        asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(acc));
     }
+    return r;
 }
 
 template <int G>
@@ -105,7 +103,7 @@ __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch,
             dkk = lane_value(a[k], k);
         } else {
             col[lane] = a[k];
-            __syncthreads();
+            lds_barrier();
             dkk = col[g * G + k];
         }
         const double den = sqrt(dkk);
@@ -116,7 +114,7 @@ __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch,
             a[k] = a[k] / den;
         }
         xch[lane] = a[k];
-        __syncthreads();
+        lds_barrier();
         // The multipliers are read in chunks of UC, one chunk ahead of its
         // updates; the scheduling barriers stop the compiler from hoisting
         // every read of the step (which would need 2*(NB-k) more VGPRs).
@@ -183,7 +181,7 @@ __device__ __forceinline__ void forward(const double *L, int lda, double (&b)[CH
                 double *buf = s.xch[i & 1];
 #pragma unroll
                 for (int q = 0; q < CH; ++q) buf[lane * CH + q] = b[q];
-                __syncthreads();
+                lds_barrier();
                 const double di = s.diag[g * G + i];
 #pragma unroll
                 for (int q = 0; q < CH; ++q) {
@@ -196,7 +194,7 @@ __device__ __forceinline__ void forward(const double *L, int lda, double (&b)[CH
 #pragma unroll
         for (int t = 0; t < PF; ++t) cur[t] = nxt[t];
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // One backward step i with li = L[r, i] (0 above the diagonal / outside).
@@ -210,7 +208,7 @@ __device__ __forceinline__ void backward_step(int i, double li, double (&b)[CH],
         double *acc = s.acc[i & 1];
 #pragma unroll
         for (int q = 0; q < CH; ++q) p[lane * CH + q] = li * b[q];
-        __syncthreads();
+        lds_barrier();
         const int len = n - i - 1;
         const int full = len & ~7;
         // Every load of a chain is independent of the adds, so all are issued
@@ -228,7 +226,7 @@ __device__ __forceinline__ void backward_step(int i, double li, double (&b)[CH],
                 acc[(g * 8 + r) * CH + q] = t;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (r == i && i < n) {
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
@@ -288,7 +286,7 @@ __device__ __forceinline__ void backward(const double *L, int lda, double (&b)[C
 #pragma unroll
         for (int t = 0; t < PF; ++t) cur[t] = nxt[t];
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // ------------------------------------------------------------ fused factor + solve
@@ -412,10 +410,10 @@ __device__ __forceinline__ void forward_reg(const double (&a)[NB], double (&b)[C
 #pragma unroll
                 for (int q = 0; q < CH; ++q) s.x[g][q] = t[q];
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int q = 0; q < CH; ++q) y[q] = s.x[g][q];
-            __syncthreads();
+            lds_barrier();
         }
         if (r == i) {
 #pragma unroll
@@ -445,7 +443,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
 #pragma unroll
             for (int q = 0; q < CH; ++q) s.bi[g][q] = b[q];
         }
-        __syncthreads();
+        lds_barrier();
         const int len = n - i - 1;
         const int full = len > 0 ? (len & ~7) : 0;
         const int pg = g * G + i + 1;  // first row of the dot in s.p
@@ -465,7 +463,7 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
                 s.acc[g][idx] = acc;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (r < CH) {
             const int q = r;
             double res = 0.0;
@@ -495,12 +493,12 @@ __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[
             }
             s.x[g][q] = xq;
         }
-        __syncthreads();
+        lds_barrier();
         if (r == i) {
 #pragma unroll
             for (int q = 0; q < CH; ++q) b[q] = s.x[g][q];
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -544,7 +542,7 @@ __device__ __forceinline__ Member member(const int *order, int begin, int end, c
 
 // Factorisation: one launch per size bin, the row array in registers.
 template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu<NB>(), 8))) void potrf_kernel(
+__global__ __launch_bounds__(64) void potrf_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
     double *const *__restrict__ Aptr, int *__restrict__ info, double *A1, int n1, int lda1)
 {
@@ -567,7 +565,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_per_eu
 // so many wavefronts hide the latency of the step-by-step dependence chains.
 // Members whose factorisation failed keep their right-hand sides.
 template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void potrs_kernel(
+__global__ __launch_bounds__(64) void potrs_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
     double *const *__restrict__ Aptr, double *const *__restrict__ Bptr, int nrhs, const int *__restrict__ info,
     double *A1, double *B1, int n1, int lda1, int ldb1)
@@ -580,7 +578,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const int fail = mb.valid ? info[order ? mb.jm : 0] : 1;
     const int n = mb.n;
     s.diag[lane] = (r < n) ? mb.A[r + (size_t)r * mb.lda] : 1.0;
-    __syncthreads();
+    lds_barrier();
     for (int c0 = 0; c0 < nrhs; c0 += CH) {
         double b[CH];
 #pragma unroll
@@ -624,7 +622,7 @@ __global__ __launch_bounds__(64) void potrf_potrs_kernel(const int *__restrict__
     const bool mydok = div_den_ok(mydiag);
     s.dg[lane] = mydiag;
     s.inv[lane] = myinv;
-    __syncthreads();
+    lds_barrier();
     const bool care = mb.valid && fail == 0;
     for (int c0 = 0; c0 < nrhs; c0 += CH) {
         const int qn = nrhs - c0 < CH ? nrhs - c0 : CH;
@@ -645,7 +643,7 @@ __global__ __launch_bounds__(64) void potrf_potrs_kernel(const int *__restrict__
 // Adjoint of X = A^{-1} B given L: Lambda = (L L^T)^{-1} dX (same arithmetic as
 // Cholesky::solve_mut), dB = Lambda, dA = -1/2 (Lambda X^T + X Lambda^T).
 template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void potrs_backward_kernel(
+__global__ __launch_bounds__(64) void potrs_backward_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
     const double *const *__restrict__ Lptr, const double *const *__restrict__ Xptr,
     const double *const *__restrict__ dXptr, double *const *__restrict__ dAptr,
@@ -669,7 +667,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 
     // the diagonal of L, published for the substitutions; padding rows get 1
     s.diag[lane] = (r < n) ? L[r + (size_t)r * n] : 1.0;
-    __syncthreads();
+    lds_barrier();
 
     for (int c0 = 0; c0 < nrhs; c0 += CH) {
         double b[CH], x[CH];
@@ -692,7 +690,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             lam_s[0][lane * CH + q] = b[q];
             lam_s[1][lane * CH + q] = x[q];
         }
-        __syncthreads();
+        lds_barrier();
         const int nq = nrhs - c0 < CH ? nrhs - c0 : CH;
         if (r < n) {
             for (int c = 0; c < n; ++c) {
@@ -708,7 +706,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                 dA[r + (size_t)c * n] = v;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 

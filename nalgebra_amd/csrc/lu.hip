@@ -155,7 +155,7 @@ int getrf_impl(nrocm_handle h, hipStream_t st, int m, int n, T *A, int lda, int 
         // the panel's swaps on every column outside it, through a composed row plan
         if (j0 > 0 || j1 < n) {
             int *plan = (int *)h->small + NROCM_SMALL_PLAN;
-            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(PLAN_MAX), 0, st, ipiv, j0, j1, plan);
+            hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(64), 0, st, ipiv, j0, j1, plan);
             LU_TRY(hipGetLastError());
             const int ncol = j0 + (n - j1);
             hipLaunchKernelGGL((apply_plan_kernel<T>), dim3(nrocm_cdiv(ncol, 2)), dim3(256), 0, st, A, lda, plan, 0,

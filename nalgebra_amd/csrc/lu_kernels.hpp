@@ -27,18 +27,63 @@ __host__ __device__ constexpr int panel_width(int rpt, int elem_bytes)
                            : (rpt <= 2 ? 32 : rpt == 4 ? 16 : rpt == 8 ? 8 : rpt == 16 ? 4 : 2);
 }
 
-// pivot candidate: |value| and global row; `first-index strict max` order.
-struct Cand {
-    double v;
-    int idx;
-};
-
 // a beats b under icamax's rule (src/base/min_max.rs:221-241): larger |x|,
 // ties to the lower index.  NaN never wins except at the column head, which
 // the caller encodes as +inf with the lowest index.
-__device__ __forceinline__ bool beats(double av, int ai, double bv, int bi)
+template <typename V>
+__device__ __forceinline__ bool beats_v(V av, int ai, V bv, int bi)
 {
     return av > bv || (av == bv && ai < bi);
+}
+
+// 32-bit DPP move with `old` kept where the source lane is outside the pattern.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp32(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xf, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double old, double v)
+{
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = dpp32<CTRL, ROWMASK>((int)o, (int)x);
+    const int hi = dpp32<CTRL, ROWMASK>((int)(o >> 32), (int)(x >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float old, float v)
+{
+    return __int_as_float(dpp32<CTRL, ROWMASK>(__float_as_int(old), __float_as_int(v)));
+}
+
+template <typename V, int CTRL, int ROWMASK>
+__device__ __forceinline__ void argmax_dpp_step(V &bv, int &bi)
+{
+    V ov;
+    if constexpr (sizeof(V) == 8) ov = dpp_f64<CTRL, ROWMASK>(V(-1), bv);
+    else ov = dpp_f32<CTRL, ROWMASK>(V(-1), bv);
+    const int oi = dpp32<CTRL, ROWMASK>(0x7fffffff, bi);
+    if (beats_v(ov, oi, bv, bi)) {
+        bv = ov;
+        bi = oi;
+    }
+}
+
+// Wavefront argmax under the first-index strict-max rule; lane 63 ends with
+// the result.  Row scans by row_shr 1, 2, 4, 8, then row_bcast 15 into rows 1
+// and 3 and row_bcast 31 into rows 2 and 3.  The rule is a total order on
+// (value, row) pairs, so the combination order does not change the winner.
+template <typename V>
+__device__ __forceinline__ void argmax_wave(V &bv, int &bi)
+{
+    argmax_dpp_step<V, 0x111, 0xf>(bv, bi);
+    argmax_dpp_step<V, 0x112, 0xf>(bv, bi);
+    argmax_dpp_step<V, 0x114, 0xf>(bv, bi);
+    argmax_dpp_step<V, 0x118, 0xf>(bv, bi);
+    argmax_dpp_step<V, 0x142, 0xa>(bv, bi);
+    argmax_dpp_step<V, 0x143, 0xc>(bv, bi);
 }
 
 // Returns v unchanged.  Copy loops whose values pass through this are not
@@ -67,7 +112,7 @@ template <typename T, int RPT, int W>
 __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(T *A, int lda, int m, int col0, int w,
                                                               int *ipiv, int *info)
 {
-    __shared__ double red_v[PANEL_THREADS / 64];
+    __shared__ T red_v[PANEL_THREADS / 64];
     __shared__ int red_i[PANEL_THREADS / 64];
     __shared__ T rowbuf[2][2][W];  // [parity][0: pivot row, 1: row i]
     const int t = threadIdx.x;
@@ -89,46 +134,46 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(T *A, int lda, int
         if (c < w) {
             const int i = row0 + c;  // pivot row position (global)
             // ---- pivot search over rows i..m-1 of column c
-            double bv = -1.0;
+            T bv = T(-1);
             int bi = 0x7fffffff;
 #pragma unroll
             for (int s = 0; s < RPT; ++s) {
                 const int r = row0 + t + PANEL_THREADS * s;
-                if (r >= i && r < m) {
-                    double v = fabs((double)x[s][c]);
-                    if (v != v) v = (r == i) ? __builtin_inf() : -1.0;  // NaN rule
-                    if (beats(v, r, bv, bi)) { bv = v; bi = r; }
-                }
+                T v = fabs(x[s][c]);
+                v = (v != v) ? ((r == i) ? T(__builtin_inf()) : T(-1)) : v;  // NaN rule
+                v = (r >= i && r < m) ? v : T(-1);
+                // a thread visits its rows in increasing order: a tie keeps the lower row
+                if (v > bv) { bv = v; bi = r; }
             }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const double ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(bi, off);
-                if (beats(ov, oi, bv, bi)) { bv = ov; bi = oi; }
-            }
-            if (lane == 0) { red_v[wv] = bv; red_i[wv] = bi; }
-            __syncthreads();
+            argmax_wave<T>(bv, bi);
+            if (lane == 63) { red_v[wv] = bv; red_i[wv] = bi; }
+            lds_barrier();
             bv = red_v[0];
             bi = red_i[0];
 #pragma unroll
             for (int q = 1; q < PANEL_THREADS / 64; ++q)
-                if (beats(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
+                if (beats_v(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
             const int piv = bi;
-            // ---- exchange rows i and piv through LDS
+            // ---- exchange rows i and piv through LDS (only their owners copy)
             T(*rb)[W] = rowbuf[c & 1];
+            const int opiv = piv - row0 - t, oi = i - row0 - t;  // multiples of PANEL_THREADS for the owners
+            if (opiv >= 0 && opiv % PANEL_THREADS == 0) {
 #pragma unroll
-            for (int s = 0; s < RPT; ++s) {
-                const int r = row0 + t + PANEL_THREADS * s;
-                if (r == piv) {
+                for (int s = 0; s < RPT; ++s)
+                    if (opiv == PANEL_THREADS * s) {
 #pragma unroll
-                    for (int q = 0; q < W; ++q) rb[0][q] = pass(x[s][q]);
-                }
-                if (r == i) {
-#pragma unroll
-                    for (int q = 0; q < W; ++q) rb[1][q] = pass(x[s][q]);
-                }
+                        for (int q = 0; q < W; ++q) rb[0][q] = pass(x[s][q]);
+                    }
             }
-            __syncthreads();
+            if (oi >= 0 && oi % PANEL_THREADS == 0) {
+#pragma unroll
+                for (int s = 0; s < RPT; ++s)
+                    if (oi == PANEL_THREADS * s) {
+#pragma unroll
+                        for (int q = 0; q < W; ++q) rb[1][q] = pass(x[s][q]);
+                    }
+            }
+            lds_barrier();
             const T d = rb[0][c];  // the pivot value
             if (t == 0) ipiv[col0 + c] = (d == T(0)) ? i : piv;
             if (d == T(0)) {
@@ -136,16 +181,26 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(T *A, int lda, int
                 if (t == 0 && info && *info == 0) *info = i + 1;
             } else {
                 const T inv = T(1) / d;
+                // rows i and piv trade places (their owners only)
+                if (oi >= 0 && oi % PANEL_THREADS == 0) {
+#pragma unroll
+                    for (int s = 0; s < RPT; ++s)
+                        if (oi == PANEL_THREADS * s) {
+#pragma unroll
+                            for (int q = 0; q < W; ++q) x[s][q] = pass(rb[0][q]);
+                        }
+                }
+                if (piv != i && opiv >= 0 && opiv % PANEL_THREADS == 0) {
+#pragma unroll
+                    for (int s = 0; s < RPT; ++s)
+                        if (opiv == PANEL_THREADS * s) {
+#pragma unroll
+                            for (int q = 0; q < W; ++q) x[s][q] = pass(rb[1][q]);
+                        }
+                }
 #pragma unroll
                 for (int s = 0; s < RPT; ++s) {
                     const int r = row0 + t + PANEL_THREADS * s;
-                    if (r == i) {
-#pragma unroll
-                        for (int q = 0; q < W; ++q) x[s][q] = pass(rb[0][q]);
-                    } else if (r == piv) {
-#pragma unroll
-                        for (int q = 0; q < W; ++q) x[s][q] = pass(rb[1][q]);
-                    }
                     if (r > i && r < m) {
                         const T l = x[s][c] * inv;
                         x[s][c] = l;
@@ -374,51 +429,47 @@ __global__ __launch_bounds__(256) void panel_cols_kernel(T *A, int lda, int m, i
 // plan[1 + s] holds the value that was at position plan[1 + PLAN_MAX + s].
 constexpr int PLAN_MAX = 128;
 
-__global__ __launch_bounds__(128) void swap_plan_kernel(const int *ipiv, int k0, int k1, int *plan)
+// One wavefront; lane l keeps slots l and l + 64 of the plan in registers, and
+// the slots of positions i and p are found by ballot, so no barriers are needed.
+__global__ __launch_bounds__(64) void swap_plan_kernel(const int *ipiv, int k0, int k1, int *plan)
 {
-    __shared__ int pos[PLAN_MAX], who[PLAN_MAX], pv[PLAN_MAX / 2];
-    __shared__ int cnt, sa, sb;
-    const int t = threadIdx.x;
-    if (t < k1 - k0) pv[t] = ipiv[k0 + t];
-    if (t == 0) cnt = 0;
-    __syncthreads();
+    const int l = threadIdx.x;
+    int pos0 = -1, who0 = -1, pos1 = -1, who1 = -1;
+    const int myp = (l < k1 - k0) ? ipiv[k0 + l] : 0;
+    int cnt = 0;
     for (int i = k0; i < k1; ++i) {
-        const int p = pv[i - k0];
+        const int p = __shfl(myp, i - k0);
         if (p == i) continue;  // uniform: no swap at this step
-        if (t == 0) {
-            sa = -1;
-            sb = -1;
-        }
-        __syncthreads();
-        if (t < cnt) {
-            const int ps = pos[t];  // positions are distinct: at most one match each
-            if (ps == i) sa = t;
-            if (ps == p) sb = t;
-        }
-        __syncthreads();
-        if (t == 0) {
-            int a = sa, b = sb;
-            if (a < 0) {
-                a = cnt++;
-                pos[a] = i;
-                who[a] = i;
+        const unsigned long long ma0 = __ballot(pos0 == i), ma1 = __ballot(pos1 == i);
+        const unsigned long long mb0 = __ballot(pos0 == p), mb1 = __ballot(pos1 == p);
+        int a = ma0 ? __builtin_ctzll(ma0) : ma1 ? 64 + __builtin_ctzll(ma1) : -1;
+        int b = mb0 ? __builtin_ctzll(mb0) : mb1 ? 64 + __builtin_ctzll(mb1) : -1;
+        if (a < 0) {
+            a = cnt++;
+            if (l == (a & 63)) {
+                if (a < 64) { pos0 = i; who0 = i; } else { pos1 = i; who1 = i; }
             }
-            if (b < 0) {
-                b = cnt++;
-                pos[b] = p;
-                who[b] = p;
-            }
-            const int tw = who[a];
-            who[a] = who[b];
-            who[b] = tw;
         }
-        __syncthreads();
+        if (b < 0) {
+            b = cnt++;
+            if (l == (b & 63)) {
+                if (b < 64) { pos0 = p; who0 = p; } else { pos1 = p; who1 = p; }
+            }
+        }
+        // swap who[a] and who[b]
+        const int wa = __shfl(a < 64 ? who0 : who1, a & 63);
+        const int wb = __shfl(b < 64 ? who0 : who1, b & 63);
+        if (l == (a & 63)) { if (a < 64) who0 = wb; else who1 = wb; }
+        if (l == (b & 63)) { if (b < 64) who0 = wa; else who1 = wa; }
     }
-    const int nt = cnt;
-    if (t == 0) plan[0] = nt;
-    if (t < nt) {
-        plan[1 + t] = pos[t];
-        plan[1 + PLAN_MAX + t] = who[t];
+    if (l == 0) plan[0] = cnt;
+    if (l < cnt) {
+        plan[1 + l] = pos0;
+        plan[1 + PLAN_MAX + l] = who0;
+    }
+    if (64 + l < cnt) {
+        plan[1 + 64 + l] = pos1;
+        plan[1 + PLAN_MAX + 64 + l] = who1;
     }
 }
 

@@ -10,6 +10,20 @@
 
 #include "../../include/nalgebra_rocm.h"
 
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() also waits
+// for every outstanding global load and store of the calling wavefront, which
+// stalls kernels that keep global stores in flight across their barriers.
+// Use this only where threads exchange data through LDS, not global memory.
+// (This header is also read by the host-only runtime, built without hipcc.)
+#ifdef __HIPCC__
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+#endif
+
 constexpr int NROCM_SMALL_PLAN = 256;  // int offset of the LU row plan (257 ints) in small
 
 struct nrocm_handle_s {
