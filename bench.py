@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--count", type=int, default=65536)
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-autotune", action="store_true", help="batched Cholesky: keep the default schedule")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--reps-lu", type=int, default=3)
@@ -111,6 +112,10 @@ def bench_chol(args):
     bt = nl.CholeskyBatch(ns, nrhs, device=dev)
     ptrs = [bt.pointers(a, b) for a, b in copies]
     infos = [torch.empty(len(ns), dtype=torch.int32, device=dev) for _ in range(K)]
+    # schedule per size bin: measured by the library's autotuner unless disabled
+    t_tune = time.perf_counter()
+    schedule = bt.schedule(mode) if args.no_autotune else bt.autotune(mode)
+    t_tune = time.perf_counter() - t_tune
 
     # warm-up: capture one graph per copy, then restore the inputs
     for i in range(max(args.warmup, K)):
@@ -169,7 +174,9 @@ def bench_chol(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "batched_cholesky_solve", "count": int(len(ns)), "n_range": [4, 64],
-                   "nrhs": nrhs, "mode": args.mode, "seed": args.seed, "graph": True},
+                   "nrhs": nrhs, "mode": args.mode, "seed": args.seed, "graph": True,
+                   "lanes_per_matrix": {str(k): v for k, v in schedule.items()},
+                   "autotuned": not args.no_autotune, "autotune_s": round(t_tune, 3)},
         "roofline": {"bound": "hbm", "achieved": alg_bytes / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg_bytes / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "alg_bytes_per_step": alg_bytes},

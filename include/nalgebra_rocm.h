@@ -105,9 +105,10 @@ int nrocm_dpotrs(nrocm_handle handle, int n, int nrhs, const double *l, int lda,
 /* ---------------------------------------------------------------- batched, uneven sizes
  * A batch describes `count` independent systems whose sizes are given on the
  * host (n_host[j], or m_host[j] for least squares).  Creating it sorts the
- * members by size, bins them, consults the tuning table for each bin's
- * schedule and uploads the schedule; calls that use it may be captured in a
- * hipGraph (done internally on repeated calls with the same pointers).
+ * members by size, bins them (padded sizes 8, 16, ..., 64) and gives each
+ * bin a default lanes-per-matrix schedule (nrocm_batch_autotune replaces it
+ * by a measured one); calls that use it may be captured in a hipGraph (done
+ * internally on repeated calls with the same pointers).
  * No reference counterpart: the reference solves a batch as a loop of
  * Cholesky::new + solve_mut. */
 int nrocm_batch_create(nrocm_handle handle, int count, const int *n_host, int nrhs,
@@ -115,6 +116,22 @@ int nrocm_batch_create(nrocm_handle handle, int count, const int *n_host, int nr
 int nrocm_batch_destroy(nrocm_batch batch);
 /* Use (1) or stop using (0) hipGraph capture + replay for this batch. */
 int nrocm_batch_set_graph(nrocm_batch batch, int enable);
+/* Autotuner of the batched Cholesky schedule (no reference counterpart).
+ * For each size bin of the batch, times every compiled lanes-per-matrix
+ * schedule (8, 16, 32 or 64 lanes, i.e. 8 to 1 matrices per wavefront) of
+ * the fused factor+solve kernel in `mode` on up to 4096 generated SPD members
+ * of that bin's sizes, and keeps the fastest for later
+ * nrocm_dpotrf_potrs_batched calls in that mode.  The caller's matrices are
+ * not touched.  Synchronises the handle's stream.  lanes (host, 8 entries or
+ * NULL) receives the chosen lanes per padded size 8, 16, ..., 64 (0 where
+ * the batch has no member of that size class). */
+int nrocm_batch_autotune(nrocm_handle handle, nrocm_batch batch, int mode, int *lanes);
+/* Current lanes-per-matrix schedule of `mode` in the same format. */
+int nrocm_batch_schedule(nrocm_batch batch, int mode, int *lanes);
+/* Sets the schedule of `mode` (same format; 0 keeps a size class's current
+ * choice).  Each entry must be 8, 16, 32 or 64 and at least the padded size;
+ * otherwise -3 and nothing changes. */
+int nrocm_batch_set_schedule(nrocm_batch batch, int mode, const int *lanes);
 
 /* For each j: Cholesky::new(A_j) then solve_mut(B_j), in one persistent
  * launch.  a[j] -> n_j x n_j (lda n_j), b[j] -> n_j x nrhs (ldb n_j); a and b

@@ -51,6 +51,9 @@ _PROTOS = {
     "nrocm_batch_create": (_i, [_vp, _i, _ip, _i, C.POINTER(_vp)]),
     "nrocm_batch_destroy": (_i, [_vp]),
     "nrocm_batch_set_graph": (_i, [_vp, _i]),
+    "nrocm_batch_autotune": (_i, [_vp, _vp, _i, _vp]),
+    "nrocm_batch_schedule": (_i, [_vp, _i, _vp]),
+    "nrocm_batch_set_schedule": (_i, [_vp, _i, _vp]),
     "nrocm_dpotrf_potrs_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
     "nrocm_dpotrs_backward_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nrocm_dgeqrf": (_i, [_vp, _i, _i, _vp, _i, _vp]),

@@ -59,6 +59,26 @@ class CholeskyBatch:
     def set_graph(self, enable: bool) -> None:
         check("nrocm_batch_set_graph", lib().nrocm_batch_set_graph(self._batch, int(enable)))
 
+    def autotune(self, mode: int = EXACT) -> dict:
+        """Measure and keep the fastest lanes-per-matrix schedule of every size
+        bin (nrocm_batch_autotune).  Returns {padded size: lanes per matrix}."""
+        lanes = (C.c_int * 8)()
+        h = handle()
+        check("nrocm_batch_autotune", lib().nrocm_batch_autotune(h, self._batch, mode, lanes), h)
+        return {8 * (c + 1): lanes[c] for c in range(8) if lanes[c]}
+
+    def set_schedule(self, lanes: dict, mode: int = EXACT) -> None:
+        """lanes: {padded size: lanes per matrix} (nrocm_batch_set_schedule)."""
+        arr = (C.c_int * 8)()
+        for nb, g in lanes.items():
+            arr[nb // 8 - 1] = int(g)
+        check("nrocm_batch_set_schedule", lib().nrocm_batch_set_schedule(self._batch, mode, arr))
+
+    def schedule(self, mode: int = EXACT) -> dict:
+        lanes = (C.c_int * 8)()
+        check("nrocm_batch_schedule", lib().nrocm_batch_schedule(self._batch, mode, lanes))
+        return {8 * (c + 1): lanes[c] for c in range(8) if lanes[c]}
+
     # -------------------------------------------------------------- packed storage
     def alloc(self):
         A = torch.empty(self.sizeA, dtype=torch.float64, device=self.device)

@@ -31,8 +31,8 @@ using SolLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, co
 using BwdLaunch = int (*)(unsigned, hipStream_t, const int *, int, int, const int *,
                          const double *const *, const double *const *, const double *const *,
                          double *const *, double *const *, int);
-using FusedLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
-                           double *const *, double *const *, int, int *);
+using FusedLaunch = int (*)(bool, int, hipStream_t, const int *, int, int, const int *, double *const *,
+                           double *const *, int, int *);
 
 const FacLaunch kFac[8] = {nrocm_chol_potrf_launch_8,  nrocm_chol_potrf_launch_16,
                            nrocm_chol_potrf_launch_24, nrocm_chol_potrf_launch_32,
@@ -52,12 +52,14 @@ const FusedLaunch kFused[8] = {nrocm_chol_fused_launch_8,  nrocm_chol_fused_laun
                                nrocm_chol_fused_launch_40, nrocm_chol_fused_launch_48,
                                nrocm_chol_fused_launch_56, nrocm_chol_fused_launch_64};
 
-int mpw_of_bin(int bin)
+// Default lanes per matrix: the smallest power-of-two group holding the rows.
+int default_lanes(int bin)
 {
     const int nb = 8 * (bin + 1);
-    const int G = nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
-    return 64 / G;
+    return nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
 }
+
+int mpw_of_bin(int bin) { return 64 / default_lanes(bin); }
 
 int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const *A,
                    double *const *B, int *info, bool exact)
@@ -66,12 +68,152 @@ int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const
         const int items = bin.end - bin.begin;
         if (items <= 0) continue;
         const int mpw = mpw_of_bin(bin.kernel);
-        const unsigned grid = nrocm_cdiv(items, mpw);
-        if (kFused[bin.kernel](exact, grid, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
+        (void)mpw;
+        const int G = bt->lanes[exact ? 0 : 1][&bin - bt->bins.data()];
+        if (kFused[bin.kernel](exact, G, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
                                bt->nrhs > 0 ? B : nullptr, bt->nrhs, info))
             NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;
+}
+
+// ------------------------------------------------------------------ autotuning
+
+// Deterministic strictly diagonally dominant SPD test members for timing:
+// A_rc = A_cr = h(r, c) / n with |h| <= 1/2, A_rr = n + 1 + h(r, r) / n, and
+// B entries in [-1, 1).  One workgroup per member.
+__device__ __forceinline__ double tune_hash(unsigned a, unsigned b, unsigned c)
+{
+    unsigned x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (c + 0x165667B1u) * 0xC2B2AE3Du;
+    x ^= x >> 15;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    x *= 0x297A2D39u;
+    x ^= x >> 15;
+    return (double)(x >> 8) * 0x1p-24 - 0.5;  // [-0.5, 0.5)
+}
+
+__global__ void tune_fill_kernel(const int *nsz, double *const *Aptr, double *const *Bptr, int nrhs)
+{
+    const int j = blockIdx.x;
+    const int n = nsz[j];
+    double *A = Aptr[j];
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int r = e % n, c = e / n;
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        const double off = tune_hash((unsigned)j, (unsigned)lo, (unsigned)hi) / n;
+        A[e] = (r == c) ? (double)(n + 1) + off : off;
+    }
+    if (Bptr)
+        for (int e = threadIdx.x; e < n * nrhs; e += blockDim.x)
+            Bptr[j][e] = 2.0 * tune_hash((unsigned)j, 0x10000u + (unsigned)e, 7u);
+}
+
+// Times every compiled lanes-per-matrix schedule of each bin on up to
+// kTuneSample members of that bin's sizes (generated, so the caller's data
+// is untouched) and keeps the fastest.
+int autotune(nrocm_handle h, nrocm_batch bt, bool exact)
+{
+    constexpr int kTuneSample = 4096, kReps = 3;
+    const int mi = exact ? 0 : 1;
+    hipStream_t st = h->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    NROCM_CHECK_HIP(h, hipEventCreate(&e0));
+    NROCM_CHECK_HIP(h, hipEventCreate(&e1));
+    int status = NROCM_OK;
+    for (size_t bi = 0; bi < bt->bins.size() && status == NROCM_OK; ++bi) {
+        const auto &bin = bt->bins[bi];
+        const int cand[4] = {8, 16, 32, 64};
+        int ncand = 0;
+        for (int G : cand) ncand += (G >= bin.nb) ? 1 : 0;
+        if (ncand < 2) continue;
+        const int k = std::min(bin.end - bin.begin, kTuneSample);
+        std::vector<int> ns(k);
+        std::vector<size_t> offA(k), offB(k);
+        size_t totA = 0, totB = 0;
+        for (int q = 0; q < k; ++q) {
+            ns[q] = bt->n_host[bt->order_host[bin.begin + q]];
+            offA[q] = totA;
+            offB[q] = totB;
+            totA += (size_t)ns[q] * ns[q];
+            totB += (size_t)ns[q] * bt->nrhs;
+        }
+        double *dA = nullptr, *dB = nullptr;
+        int *dn = nullptr, *dord = nullptr, *dinfo = nullptr;
+        double **pA = nullptr, **pB = nullptr;
+        std::vector<double *> hpA(k), hpB(k);
+        std::vector<int> ord(k);
+        auto cleanup = [&]() {
+            (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dn); (void)hipFree(dord);
+            (void)hipFree(dinfo); (void)hipFree(pA); (void)hipFree(pB);
+        };
+        if (hipMalloc(&dA, totA * sizeof(double)) != hipSuccess ||
+            (totB && hipMalloc(&dB, totB * sizeof(double)) != hipSuccess) ||
+            hipMalloc(&dn, k * sizeof(int)) != hipSuccess || hipMalloc(&dord, k * sizeof(int)) != hipSuccess ||
+            hipMalloc(&dinfo, k * sizeof(int)) != hipSuccess || hipMalloc(&pA, k * sizeof(double *)) != hipSuccess ||
+            hipMalloc(&pB, k * sizeof(double *)) != hipSuccess) {
+            cleanup();
+            status = nrocm_fail(h, NROCM_ERR_ALLOC, "autotune scratch allocation failed");
+            break;
+        }
+        for (int q = 0; q < k; ++q) {
+            hpA[q] = dA + offA[q];
+            hpB[q] = totB ? dB + offB[q] : nullptr;
+            ord[q] = q;
+        }
+        double *const *Bp = totB ? pB : nullptr;
+        hipError_t e = hipMemcpy(dn, ns.data(), k * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dord, ord.data(), k * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(pA, hpA.data(), k * sizeof(double *), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(pB, hpB.data(), k * sizeof(double *), hipMemcpyHostToDevice);
+        float best = 0.0f;
+        int bestG = bt->lanes[mi][bi];
+        for (int G : cand) {
+            if (G < bin.nb || e != hipSuccess) continue;
+            float tmin = 1e30f;
+            for (int rep = 0; rep <= kReps && e == hipSuccess; ++rep) {  // rep 0 warms up
+                hipLaunchKernelGGL(tune_fill_kernel, dim3(k), dim3(256), 0, st, dn, pA, Bp, bt->nrhs);
+                e = hipGetLastError();
+                if (e == hipSuccess) e = hipEventRecord(e0, st);
+                if (e == hipSuccess && kFused[bin.kernel](exact, G, st, dord, 0, k, dn, pA, Bp, bt->nrhs, dinfo))
+                    e = hipErrorLaunchFailure;
+                if (e == hipSuccess) e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e == hipSuccess && rep > 0) tmin = std::min(tmin, ms);
+            }
+            if (e == hipSuccess && (best == 0.0f || tmin < best)) {
+                best = tmin;
+                bestG = G;
+            }
+        }
+        cleanup();
+        if (e != hipSuccess) {
+            status = nrocm_fail_hip(h, e, "autotune timing");
+            break;
+        }
+        bt->lanes[mi][bi] = bestG;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return status;
+}
+
+void drop_graphs(nrocm_batch bt)
+{
+    for (auto &g : bt->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    bt->graphs.clear();
+}
+
+void report_lanes(nrocm_batch bt, int mi, int *lanes)
+{
+    if (!lanes) return;
+    for (int c = 0; c < 8; ++c) lanes[c] = 0;
+    for (size_t bi = 0; bi < bt->bins.size(); ++bi) lanes[bt->bins[bi].kernel] = bt->lanes[mi][bi];
 }
 
 }  // namespace
@@ -126,6 +268,8 @@ int nrocm_batch_create(nrocm_handle h, int count, const int *n_host, int nrhs, n
         while (endp < count && std::max(0, (n_host[bt->order_host[endp]] + 7) / 8 - 1) == bin)
             ++endp;
         bt->bins.push_back({8 * (bin + 1), pos, endp, bin});
+        bt->lanes[0].push_back(default_lanes(bin));
+        bt->lanes[1].push_back(default_lanes(bin));
         pos = endp;
     }
     if (count > 0) {
@@ -148,10 +292,7 @@ int nrocm_batch_create(nrocm_handle h, int count, const int *n_host, int nrhs, n
 int nrocm_batch_destroy(nrocm_batch bt)
 {
     if (!bt) return -1;
-    for (auto &g : bt->graphs) {
-        (void)hipGraphExecDestroy(g.exec);
-        (void)hipGraphDestroy(g.graph);
-    }
+    drop_graphs(bt);
     if (bt->graph_stream) (void)hipStreamDestroy(bt->graph_stream);
     if (bt->order_dev) (void)hipFree(bt->order_dev);
     if (bt->n_dev) (void)hipFree(bt->n_dev);
@@ -163,6 +304,45 @@ int nrocm_batch_set_graph(nrocm_batch bt, int enable)
 {
     if (!bt) return -1;
     bt->use_graph = enable != 0;
+    return NROCM_OK;
+}
+
+int nrocm_batch_autotune(nrocm_handle h, nrocm_batch bt, int mode, int *lanes)
+{
+    if (!h) return -1;
+    if (!bt || bt->handle != h) return -2;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -3;
+    if (bt->max_n > 64)
+        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "batched Cholesky supports n <= 64");
+    NROCM_CHECK_HIP(h, hipStreamSynchronize(h->stream));
+    const int st = autotune(h, bt, mode == NROCM_EXACT);
+    drop_graphs(bt);  // captured launches used the previous schedule
+    report_lanes(bt, mode == NROCM_EXACT ? 0 : 1, lanes);
+    return st;
+}
+
+int nrocm_batch_set_schedule(nrocm_batch bt, int mode, const int *lanes)
+{
+    if (!bt) return -1;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -2;
+    if (!lanes) return -3;
+    for (int c = 0; c < 8; ++c) {
+        const int G = lanes[c];
+        if (G != 0 && ((G != 8 && G != 16 && G != 32 && G != 64) || G < 8 * (c + 1))) return -3;
+    }
+    const int mi = mode == NROCM_EXACT ? 0 : 1;
+    for (size_t bi = 0; bi < bt->bins.size(); ++bi)
+        if (lanes[bt->bins[bi].kernel]) bt->lanes[mi][bi] = lanes[bt->bins[bi].kernel];
+    drop_graphs(bt);
+    return NROCM_OK;
+}
+
+int nrocm_batch_schedule(nrocm_batch bt, int mode, int *lanes)
+{
+    if (!bt) return -1;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -2;
+    if (!lanes) return -3;
+    report_lanes(bt, mode == NROCM_EXACT ? 0 : 1, lanes);
     return NROCM_OK;
 }
 

@@ -710,6 +710,39 @@ __global__ __launch_bounds__(64) void potrs_backward_kernel(
     }
 }
 
+// Launches the fused kernel for members [begin, end) of `order` with G lanes
+// per matrix (64 / G matrices per wavefront).
+template <int NB, int G>
+int fused_launch_g(bool exact, hipStream_t st, const int *order, int begin, int end, const int *nsz,
+                   double *const *A, double *const *B, int nrhs, int *info)
+{
+    constexpr int MPW = 64 / G;
+    const unsigned grid = (unsigned)((end - begin + MPW - 1) / MPW);
+    if (exact)
+        hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st, order, begin, end, nsz,
+                           A, B, nrhs, info);
+    else
+        hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st, order, begin, end, nsz,
+                           A, B, nrhs, info);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Lanes-per-matrix schedules compiled for padded size NB: every power of two
+// from the smallest group that holds NB rows up to a whole wavefront.
+template <int NB>
+int fused_dispatch(bool exact, int G, hipStream_t st, const int *order, int begin, int end, const int *nsz,
+                   double *const *A, double *const *B, int nrhs, int *info)
+{
+    if constexpr (NB <= 8)
+        if (G == 8) return fused_launch_g<NB, 8>(exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    if constexpr (NB <= 16)
+        if (G == 16) return fused_launch_g<NB, 16>(exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    if constexpr (NB <= 32)
+        if (G == 32) return fused_launch_g<NB, 32>(exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    if (G == 64) return fused_launch_g<NB, 64>(exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    return 2;  // no such schedule for this size
+}
+
 }  // namespace nrocm_chol
 
 #define NROCM_CHOL_DECLARE(NB)                                                                    \
@@ -724,7 +757,7 @@ __global__ __launch_bounds__(64) void potrs_backward_kernel(
                                    int end, const int *nsz, const double *const *L,               \
                                    const double *const *X, const double *const *dX,               \
                                    double *const *dA, double *const *dB, int nrhs);               \
-    int nrocm_chol_fused_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+    int nrocm_chol_fused_launch_##NB(bool exact, int G, hipStream_t st, const int *order,         \
                                      int begin, int end, const int *nsz, double *const *A,        \
                                      double *const *B, int nrhs, int *info);
 
@@ -769,19 +802,12 @@ __global__ __launch_bounds__(64) void potrs_backward_kernel(
                            order, begin, end, nsz, L, X, dX, dA, dB, nrhs);                       \
         return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
     }                                                                                             \
-    int nrocm_chol_fused_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
+    int nrocm_chol_fused_launch_##NB(bool exact, int G, hipStream_t st, const int *order,         \
                                      int begin, int end, const int *nsz, double *const *A,        \
                                      double *const *B, int nrhs, int *info)                       \
     {                                                                                             \
-        using namespace nrocm_chol;                                                               \
-        constexpr int G = group_for<NB>();                                                        \
-        if (exact)                                                                                \
-            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st,    \
-                               order, begin, end, nsz, A, B, nrhs, info);                         \
-        else                                                                                      \
-            hipLaunchKernelGGL((potrf_potrs_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st,   \
-                               order, begin, end, nsz, A, B, nrhs, info);                         \
-        return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
+        return nrocm_chol::fused_dispatch<NB>(exact, G, st, order, begin, end, nsz, A, B, nrhs,   \
+                                              info);                                              \
     }
 
 NROCM_CHOL_DECLARE(8)

@@ -64,6 +64,9 @@ struct nrocm_batch_s {
     // Bins: [begin, end) ranges of `order` sharing one kernel instantiation.
     struct Bin { int nb; int begin; int end; int kernel; };
     std::vector<Bin> bins;
+    // Lanes per matrix of each bin's fused kernel, per mode (EXACT, FAST):
+    // the default heuristic until nrocm_batch_autotune measures them.
+    std::vector<int> lanes[2];
     int *order_dev = nullptr;       // device copy of order_host
     int *n_dev = nullptr;           // device copy of n_host (caller order)
     // hipGraph replays of captured launch sequences, keyed by their arguments

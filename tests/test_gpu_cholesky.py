@@ -47,6 +47,49 @@ def test_batched_exact_is_bitwise(nl, orc, nrhs):
     assert np.array_equal(_bits(Bg), _bits(Br)), "solution bits differ"
 
 
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+def test_every_schedule_is_bitwise(nl, orc, lanes):
+    """Each compiled lanes-per-matrix schedule (the autotuner may pick any)
+    reproduces the oracle bit for bit; size classes that do not fit `lanes`
+    keep their default."""
+    ns = NS_ALL
+    A, offA, B, offB = gen.spd_batch(21, ns, 4)
+    bt = nl.CholeskyBatch(ns, 4)
+    want = {nb: lanes for nb in range(8, 65, 8) if nb <= lanes}
+    bt.set_schedule(want, nl.EXACT)
+    got = bt.schedule(nl.EXACT)
+    for nb, g in got.items():
+        assert g == want.get(nb, g)
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
+    Ar, Br = A.copy(), B.copy()
+    orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
+    assert not info.any()
+    assert np.array_equal(_bits(Ad.cpu().numpy()), _bits(Ar))
+    assert np.array_equal(_bits(Bd.cpu().numpy()), _bits(Br))
+
+
+def test_autotune_keeps_parity(nl, orc):
+    ns = NS_ALL
+    A, offA, B, offB = gen.spd_batch(22, ns, 4)
+    bt = nl.CholeskyBatch(ns, 4)
+    chosen = bt.autotune(nl.EXACT)
+    print("autotuned lanes per matrix:", chosen, flush=True)
+    assert set(chosen) == {8 * ((n + 7) // 8) for n in ns}
+    for nb, g in chosen.items():
+        assert g in (8, 16, 32, 64) and g >= nb
+    assert bt.schedule(nl.EXACT) == chosen
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
+    Ar, Br = A.copy(), B.copy()
+    orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
+    assert not info.any()
+    assert np.array_equal(_bits(Ad.cpu().numpy()), _bits(Ar))
+    assert np.array_equal(_bits(Bd.cpu().numpy()), _bits(Br))
+    with pytest.raises(nl.NrocmError):
+        bt.set_schedule({64: 32}, nl.EXACT)
+
+
 def test_batched_fast_within_1e12(nl, orc):
     ns = NS_ALL
     A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST)
