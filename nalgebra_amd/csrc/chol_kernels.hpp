@@ -431,6 +431,71 @@ __device__ __forceinline__ void forward_reg(const double (&a)[NB], double (&b)[C
 // 0 + (acc0+acc4) + (acc1+acc5) + (acc2+acc6) + (acc3+acc7), then the tail).
 // Products go through LDS; 8*CH lanes per group build the accumulators and CH
 // lanes finish the CH dots and quotients.
+// 64-bit DPP row shift left by N (lane l reads lane l + N of its row of 16);
+// lanes whose source leaves the row keep 0.
+template <int N>
+__device__ __forceinline__ double row_shl(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, 0x100 + N, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), 0x100 + N, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One group per wavefront (n uniform): lanes (q, t) = (lane / 8, lane % 8),
+// lane < 8*CH, sum acc_t with a uniform trip count; the pairs acc_t + acc_t+4
+// and the ordered combine move between the 8 lanes of a right-hand side by
+// DPP, so each step has two LDS round trips instead of three.
+template <int NB, bool EXACT>
+__device__ __forceinline__ void backward_reg64(const double (&a)[NB], double (&b)[CH], FusedSmem<1> &s, int n,
+                                               bool care, int qn, int r, double mydiag, double myinv)
+{
+    const int q = r >> 3, t = r & 7;
+    for (int i = n - 1; i >= 0; --i) {
+        const double li = reg_at<NB>(a, i);
+#pragma unroll
+        for (int qq = 0; qq < CH; ++qq) s.p[qq][r] = li * b[qq];
+        if (r == i) {
+#pragma unroll
+            for (int qq = 0; qq < CH; ++qq) s.bi[0][qq] = b[qq];
+        }
+        lds_barrier();
+        const int len = n - i - 1;
+        const int full = len > 0 ? (len & ~7) : 0;
+        const int nm = full >> 3, tl = len > 0 ? len - full : 0;
+        const int row = i + 1;  // first row of the dot
+        double acc = 0.0;
+        if (q < CH)
+            for (int m = 0; m < nm; ++m) acc += s.p[q][row + t + 8 * m];
+        const double pair = acc + row_shl<4>(acc);  // lanes t < 4: acc_t + acc_t+4
+        const double p1 = row_shl<1>(pair), p2 = row_shl<2>(pair), p3 = row_shl<3>(pair);
+        const double di = lane_value(mydiag, i);
+        const double yi = lane_value(myinv, i);
+        if (q < CH && t == 0) {
+            double res = 0.0;
+            res += pair;
+            res += p1;
+            res += p2;
+            res += p3;
+            for (int m = 0; m < tl; ++m) res += s.p[q][row + full + m];
+            const double num = s.bi[0][q] - res;
+            double xq;
+            if constexpr (EXACT) {
+                xq = div_with(num, di, yi);
+                if (care && q < qn && !(div_num_ok(num) && div_den_ok(di))) xq = num / di;
+            } else {
+                xq = num * yi;
+            }
+            s.x[0][q] = xq;
+        }
+        lds_barrier();
+        if (r == i) {
+#pragma unroll
+            for (int qq = 0; qq < CH; ++qq) b[qq] = s.x[0][qq];
+        }
+    }
+}
+
 template <int NB, int G, bool EXACT>
 __device__ __forceinline__ void backward_reg(const double (&a)[NB], double (&b)[CH], FusedSmem<64 / G> &s, int n,
                                              int nmax, bool care, int qn, int g, int r, int lane)
@@ -631,7 +696,10 @@ __global__ __launch_bounds__(64) void potrf_potrs_kernel(const int *__restrict__
         for (int q = 0; q < CH; ++q)
             b[q] = (care && r < n && q < qn) ? mb.B[r + (size_t)(c0 + q) * mb.ldb] : 0.0;
         forward_reg<NB, G, EXACT>(a, b, s, n, nmax, care, qn, g, r, lane, mydiag, myinv, mydok);
-        backward_reg<NB, G, EXACT>(a, b, s, n, nmax, care, qn, g, r, lane);
+        if constexpr (G == 64)
+            backward_reg64<NB, EXACT>(a, b, s, n, care, qn, r, mydiag, myinv);
+        else
+            backward_reg<NB, G, EXACT>(a, b, s, n, nmax, care, qn, g, r, lane);
         if (care && r < n) {
 #pragma unroll
             for (int q = 0; q < CH; ++q)
