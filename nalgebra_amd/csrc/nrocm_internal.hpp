@@ -37,6 +37,13 @@ struct nrocm_handle_s {
     size_t ws_bytes = 0;
     // Small device-side scalars (counters, flags, norms).
     void *small = nullptr;          // 4 KiB: int[0] Cholesky block info, ints from NROCM_SMALL_PLAN: LU row plan
+    // Streams for independent launches that may run side by side (created on
+    // first use, default priority), with the events that fork them from and
+    // join them back into `stream`.
+    static constexpr int kAux = 4;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kAux] = {};
 
     // hipGraph cache for long launch sequences (keyed by every argument that
     // shapes the sequence), captured on a private stream, replayed on `stream`.
@@ -87,6 +94,11 @@ int nrocm_fail_hip(nrocm_handle h, hipError_t e, const char *what);
 int nrocm_fail(nrocm_handle h, int status, const char *msg);
 // Ensures at least `bytes` of workspace; synchronises the stream when it grows.
 int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr);
+// Forks the handle's auxiliary streams from its stream (nrocm_fork) and joins
+// them back (nrocm_join): work launched on aux[i] in between runs concurrently
+// with the other auxiliary streams and after everything queued before the fork.
+int nrocm_fork(nrocm_handle h);
+int nrocm_join(nrocm_handle h);
 
 #define NROCM_CHECK_HIP(h, call)                                    \
     do {                                                            \

@@ -40,6 +40,29 @@ int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr)
     return NROCM_OK;
 }
 
+int nrocm_fork(nrocm_handle h)
+{
+    if (!h->fork_ev) {
+        NROCM_CHECK_HIP(h, hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+        for (int i = 0; i < nrocm_handle_s::kAux; ++i) {
+            NROCM_CHECK_HIP(h, hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
+            NROCM_CHECK_HIP(h, hipEventCreateWithFlags(&h->join_ev[i], hipEventDisableTiming));
+        }
+    }
+    NROCM_CHECK_HIP(h, hipEventRecord(h->fork_ev, h->stream));
+    for (int i = 0; i < nrocm_handle_s::kAux; ++i) NROCM_CHECK_HIP(h, hipStreamWaitEvent(h->aux[i], h->fork_ev, 0));
+    return NROCM_OK;
+}
+
+int nrocm_join(nrocm_handle h)
+{
+    for (int i = 0; i < nrocm_handle_s::kAux; ++i) {
+        NROCM_CHECK_HIP(h, hipEventRecord(h->join_ev[i], h->aux[i]));
+        NROCM_CHECK_HIP(h, hipStreamWaitEvent(h->stream, h->join_ev[i], 0));
+    }
+    return NROCM_OK;
+}
+
 extern "C" {
 
 int nrocm_version(void) { return 100; }
@@ -75,6 +98,11 @@ int nrocm_destroy(nrocm_handle h)
         (void)hipGraphDestroy(g.graph);
     }
     if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
+    for (int i = 0; i < nrocm_handle_s::kAux; ++i) {
+        if (h->aux[i]) (void)hipStreamDestroy(h->aux[i]);
+        if (h->join_ev[i]) (void)hipEventDestroy(h->join_ev[i]);
+    }
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
     if (h->ws) (void)hipFree(h->ws);
     if (h->small) (void)hipFree(h->small);
     delete h;
