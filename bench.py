@@ -252,23 +252,34 @@ def bench_lu(args):
     bm = nl.DMatrix(b, n, 1)
     res = {}
     flops = 2.0 / 3.0 * n ** 3
-    precs = [("fp64", nl.PREC_FP64), ("fp32", nl.PREC_FP32), ("bf16x3", nl.PREC_BF16X3), ("bf16", nl.PREC_BF16)]
+    W = nl.PREC_WINDOW
+    precs = [("fp64", nl.PREC_FP64), ("fp32", nl.PREC_FP32), ("bf16x3", nl.PREC_BF16X3), ("bf16", nl.PREC_BF16),
+             ("fp64_window", nl.PREC_FP64 | W), ("fp32_window", nl.PREC_FP32 | W),
+             ("bf16x3_window", nl.PREC_BF16X3 | W), ("fp32_window_gmres", nl.PREC_FP32 | W | nl.PREC_GMRES),
+             ("bf16_window_gmres", nl.PREC_BF16 | W | nl.PREC_GMRES),
+             ("fp64_rbt", nl.PREC_FP64 | nl.PREC_RBT), ("fp32_rbt_gmres", nl.PREC_FP32 | nl.PREC_RBT | nl.PREC_GMRES)]
     for name, p in precs:
         times, iters, berr, ok = [], 0, 0.0, False
         for rep in range(args.warmup_lu + args.reps_lu):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, iters, berr, ok = nl.solve_mixed_ir(Am, bm, p, tol=1e-14, maxit=12)
+            x, iters, berr, ok = nl.solve_mixed_ir(Am, bm, p, tol=1e-14, maxit=60 if p & nl.PREC_GMRES else 12)
             torch.cuda.synchronize()
             if rep >= args.warmup_lu:
                 times.append((time.perf_counter() - t0) * 1e3)
         ms = float(np.median(times))
         res[name] = {"ms": ms, "iters": iters, "berr": berr, "converged": ok,
                      "factor_tflops_equiv": flops / (ms * 1e-3) / 1e12}
+    # speedups against the GPU FP64 LU with partial pivoting, and for the
+    # windowed and butterfly variants also against FP64 with the same pivoting
     t64 = res["fp64"]["ms"]
     for name in res:
         res[name]["speedup_vs_fp64"] = t64 / res[name]["ms"]
-    best = min((v["ms"], k) for k, v in res.items() if v["converged"] and k != "fp64")
+        for tag in ("window", "rbt"):
+            ref = res.get("fp64_" + tag, {})
+            if "_" + tag in name and ref.get("converged"):
+                res[name]["speedup_vs_fp64_" + tag] = ref["ms"] / res[name]["ms"]
+    best = min((v["ms"], k) for k, v in res.items() if v["converged"] and not k.startswith("fp64"))
     out = {
         "metric": METRIC, "value": best[0], "unit": "ms", "n_gpus": 1, "steps": args.reps_lu,
         "warmup": args.warmup_lu, "ms_per_step": best[0], "higher_is_better": False, "scaling": "weak",

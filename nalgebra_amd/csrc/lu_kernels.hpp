@@ -494,6 +494,165 @@ __global__ __launch_bounds__(256) void apply_plan_kernel(T *A, int lda, const in
     if (valid && slot < nt) A[plan[1 + slot] + (size_t)c * lda] = v;
 }
 
+// ---------------------------------------------------------------- LU without pivoting
+// (for matrices made safe to factor without row exchanges, e.g. by a random
+// butterfly transformation; used only by the mixed-precision solver)
+
+// In-place LU of the nb x nb diagonal block at (k0, k0), nb <= 64, held in LDS
+// by one workgroup.  A zero pivot sets info (first one) and the step is skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_lu_nopiv_kernel(T *A, int lda, int k0, int nb, int *info)
+{
+    __shared__ T D[64][65];
+    const int t = threadIdx.x;
+    for (int e = t; e < nb * nb; e += blockDim.x) D[e % nb][e / nb] = A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda];
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        const T d = D[c][c];
+        const int rest = nb - c - 1;
+        if (d == T(0)) {
+            if (t == 0 && info && *info == 0) *info = k0 + c + 1;
+        } else {
+            const T inv = T(1) / d;
+            for (int r = c + 1 + t; r < nb; r += blockDim.x) D[r][c] *= inv;
+            lds_barrier();
+            for (int e = t; e < rest * rest; e += blockDim.x) {
+                const int r = c + 1 + e % rest, q = c + 1 + e / rest;
+                D[r][q] = fma(-D[r][c], D[c][q], D[r][q]);
+            }
+        }
+        lds_barrier();
+    }
+    for (int e = t; e < nb * nb; e += blockDim.x) A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda] = D[e % nb][e / nb];
+}
+
+// A[r0.., k0..k0+nb) <- A[r0.., k0..k0+nb) U11^{-1} (U11 the upper triangle of
+// the factored diagonal block): one thread per row, the row in registers.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_right_upper_kernel(T *A, int lda, int k0, int nb, int r0, int m)
+{
+    __shared__ T U[64][65];
+    __shared__ T dinv[64];
+    const int t = threadIdx.x;
+    for (int e = t; e < nb * nb; e += blockDim.x) U[e % nb][e / nb] = A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda];
+    __syncthreads();
+    if (t < nb) dinv[t] = U[t][t] != T(0) ? T(1) / U[t][t] : T(0);
+    __syncthreads();
+    const int r = r0 + blockIdx.x * blockDim.x + t;
+    if (r >= m) return;
+    T x[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) x[q] = q < nb ? A[r + (size_t)(k0 + q) * lda] : T(0);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+        if (q < nb) {
+            T v = x[q];
+#pragma unroll
+            for (int p = 0; p < 64; ++p)
+                if (p < q) v = fma(-x[p], U[p][q], v);
+            x[q] = v * dinv[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 64; ++q)
+        if (q < nb) A[r + (size_t)(k0 + q) * lda] = x[q];
+}
+
+// ---------------------------------------------------------------- LU with windowed pivoting
+// (mixed-precision solver only; pivots differ from nalgebra's by design)
+//
+// Partial pivoting restricted to a window: for the nb (<= 64) columns at k0,
+// every pivot is the largest |a| among the rows [k0, kw) not yet chosen,
+// kw - k0 <= WINDOW_ROWS.  Rows below the window are eliminated afterwards by
+// A21 U11^{-1}.  With a 128-row window at least 65 candidates remain for the
+// last column of a block, which keeps |L| and the growth close to full
+// partial pivoting on the dense test matrices while the pivot search never
+// leaves one workgroup.
+constexpr int WINDOW_ROWS = 128;
+
+// One thread per window row, the row's nb values in registers.  Rows are not
+// moved during the factorisation: each thread carries the logical position of
+// its row, only the pivot row passes through LDS, and every row is written to
+// its final position at the end.  Records ipiv (absolute, LAPACK order: step
+// k0+c exchanged rows k0+c and ipiv[k0+c]) and the composed row plan of the
+// window for apply_plan_kernel.  A zero pivot column sets info (first one)
+// and is skipped.
+template <typename T>
+__global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda, int k0, int nb, int kw, int *ipiv,
+                                                                   int *info, int *plan)
+{
+    __shared__ T prow[2][64];
+    __shared__ T wv[2][2];
+    __shared__ int wi[2][2];
+    __shared__ unsigned long long wmask[2];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = kw - k0;
+    const bool live = t < wr;
+    T a[64];
+    const T *row = A + (k0 + (live ? t : 0));
+#pragma unroll
+    for (int q = 0; q < 64; ++q) a[q] = (live && q < nb) ? row[(size_t)(k0 + q) * lda] : T(0);
+    int pos = t;
+#pragma unroll
+    for (int c = 0; c < 64; ++c) {
+        if (c < nb) {  // uniform
+            const int par = c & 1;
+            T bv = (live && pos >= c) ? fabs(a[c]) : T(-1);
+            int bi = pos;
+            argmax_wave<T>(bv, bi);
+            if (lane == 63) {
+                wv[par][wave] = bv;
+                wi[par][wave] = bi;
+            }
+            lds_barrier();
+            T pv = wv[par][0];
+            int P = wi[par][0];
+            if (beats_v(wv[par][1], wi[par][1], pv, P)) {
+                pv = wv[par][1];
+                P = wi[par][1];
+            }
+            const bool zero = !(pv > T(0));
+            if (zero) {
+                P = c;  // nothing to exchange or eliminate
+                if (t == 0 && *info == 0) *info = k0 + c + 1;
+            }
+            if (t == 0) ipiv[k0 + c] = k0 + P;
+            if (pos == P) {
+#pragma unroll
+                for (int q = 0; q < 64; ++q)
+                    if (q >= c) prow[par][q] = a[q];
+            }
+            const int old = pos;
+            pos = old == P ? c : old == c ? P : old;
+            lds_barrier();
+            if (!zero && live && pos > c) {
+                const T l = a[c] * (T(1) / prow[par][c]);
+                a[c] = l;
+#pragma unroll
+                for (int q = 0; q < 64; ++q)
+                    if (q > c) a[q] = fma(-l, prow[par][q], a[q]);
+            }
+        }
+    }
+    if (live) {
+        T *dst = A + (k0 + pos);
+#pragma unroll
+        for (int q = 0; q < 64; ++q)
+            if (q < nb) dst[(size_t)(k0 + q) * lda] = a[q];
+    }
+    // row plan: the window row now at position pos came from position t
+    const bool moved = live && pos != t;
+    const unsigned long long m = __ballot(moved);
+    if (lane == 0) wmask[wave] = m;
+    lds_barrier();
+    const int before = __popcll(m & ((1ull << lane) - 1)) + (wave ? __popcll(wmask[0]) : 0);
+    if (moved) {
+        plan[1 + before] = k0 + pos;
+        plan[1 + PLAN_MAX + before] = k0 + t;
+    }
+    if (t == 0) plan[0] = __popcll(wmask[0]) + __popcll(wmask[1]);
+}
+
 __global__ void fill_int_kernel(int *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

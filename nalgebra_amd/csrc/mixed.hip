@@ -1,6 +1,8 @@
 // Mixed-precision LU + FP64 iterative refinement (nrocm_dgesv_ir).
 //
-//   A32 = fl32(A)                      working copy (FP64 kept for residuals)
+//   A32 = fl32(A)                      working copy (FP64 kept for residuals);
+//                                      with NROCM_PREC_RBT: fl32(U^T A V), U, V
+//                                      random butterflies, factored unpivoted
 //   P A32 = L U                        blocked LU, trailing updates on FP32 or
 //                                      BF16 MFMA (FP32 accumulate); FP64 mode
 //                                      factors a copy of A with FP64 MFMA
@@ -24,6 +26,7 @@ namespace {
 constexpr int BS = 64;           // triangular-solve block
 constexpr int GEMV_CS = 8;       // column splits of the residual GEMV
 constexpr long SPIN_LIMIT = 1L << 26;
+constexpr int GM_RESTART = 20;   // FGMRES cycle length
 
 __global__ void cvt_d2s_kernel(const double *__restrict__ a, int lda, float *__restrict__ o, int ldo, int n)
 {
@@ -285,6 +288,223 @@ __global__ void zero_d_kernel(double *p, int n)
     if (i < n) p[i] = 0.0;
 }
 
+// ---------------------------------------------------------------- random butterflies
+// A recursive butterfly of depth 2 on n = 4q entries mixes each group
+// {g, g+q, g+2q, g+3q} (g < q) by a 4 x 4 matrix: level 1 pairs (g, g+2q) and
+// (g+q, g+3q) with diagonals R1, S1 (n/2 each), level 2 pairs (g, g+q) and
+// (g+2q, g+3q) with R2a, S2a, R2b, S2b (n/4 each), each level (1/sqrt 2) [R S; R -S].
+// Entries are exp(rho/10), rho uniform in [-1/2, 1/2) (Baboulin et al. 2013).
+// Parameter block of one butterfly: 2n doubles; U and V take 4n.
+
+__device__ __forceinline__ double rbt_hash01(unsigned a, unsigned b)
+{
+    unsigned x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+    x ^= x >> 15;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    x *= 0x297A2D39u;
+    x ^= x >> 15;
+    return (double)x * 0x1p-32;  // [0, 1)
+}
+
+__global__ void rbt_params_kernel(double *P, int n, unsigned seed)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4 * n) P[i] = exp((rbt_hash01(seed, (unsigned)i) - 0.5) / 10.0);
+}
+
+// M (row-major 4 x 4): the butterfly restricted to group g.
+__device__ __forceinline__ void rbt_group(const double *Wp, int n, int g, double (&M)[4][4])
+{
+    const int h2 = n / 2, h4 = n / 4;
+    const double *R1 = Wp, *S1 = Wp + h2, *R2a = Wp + n, *S2a = R2a + h4, *R2b = S2a + h4, *S2b = R2b + h4;
+    const double a = R1[g], b = S1[g], e = R1[g + h4], f = S1[g + h4];
+    const double p = R2a[g], q = S2a[g], u = R2b[g], v = S2b[g];
+    const double M0[4][4] = {{p * a, q * e, p * b, q * f},
+                             {p * a, -q * e, p * b, -q * f},
+                             {u * a, v * e, -u * b, -v * f},
+                             {u * a, -v * e, -u * b, v * f}};
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) M[r][c] = 0.5 * M0[r][c];
+}
+
+// W (working precision, ld n) = U^T A V, one thread per (row group, column group).
+template <typename T>
+__global__ void rbt_matrix_kernel(const double *__restrict__ A, int lda, int n, const double *__restrict__ PU,
+                                  const double *__restrict__ PV, T *__restrict__ W)
+{
+    const int q = n / 4;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)q * q) return;
+    const int gi = (int)(idx % q), gj = (int)(idx / q);
+    double MU[4][4], MV[4][4], X[4][4], Y[4][4];
+    rbt_group(PU, n, gi, MU);
+    rbt_group(PV, n, gj, MV);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) X[r][c] = A[(gi + r * q) + (size_t)(gj + c * q) * lda];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)  // Y = X MV
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s = fma(X[r][k], MV[k][c], s);
+            Y[r][c] = s;
+        }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)  // MU^T Y
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s = fma(MU[k][r], Y[k][c], s);
+            W[(gi + r * q) + (size_t)(gj + c * q) * n] = (T)s;
+        }
+}
+
+// o = U^T r (rounded to the working precision)
+template <typename T>
+__global__ void rbt_left_kernel(const double *__restrict__ r, const double *__restrict__ PU, int n, T *__restrict__ o)
+{
+    const int q = n / 4;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= q) return;
+    double M[4][4];
+    rbt_group(PU, n, g, M);
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = r[g + k * q];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s = fma(M[k][c], v[k], s);
+        o[g + c * q] = (T)s;
+    }
+}
+
+// x += V y (FP64), with max |x_i| by ordered-bits atomicMax.
+template <typename T>
+__global__ void rbt_right_update_kernel(double *__restrict__ x, const T *__restrict__ y, const double *__restrict__ PV,
+                                        int n, unsigned long long *xmax)
+{
+    const int q = n / 4;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= q) return;
+    double M[4][4];
+    rbt_group(PV, n, g, M);
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (double)y[g + k * q];
+    unsigned long long mx = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s = fma(M[r][k], v[k], s);
+        const double xi = x[g + r * q] + s;
+        x[g + r * q] = xi;
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(xi));
+        mx = b > mx ? b : mx;
+    }
+    atomicMax(xmax, mx);
+}
+
+// ---------------------------------------------------------------- FGMRES helpers (FP64)
+
+// h[j] = sum_i V[i + j ldv] w[i], j < k: one workgroup per j, fixed-order
+// strided partial sums and a fixed tree, so results are deterministic.
+__global__ __launch_bounds__(256) void dots_kernel(const double *__restrict__ V, int ldv, const double *__restrict__ w,
+                                                   int n, double *__restrict__ hout)
+{
+    __shared__ double red[256];
+    const int j = blockIdx.x, t = threadIdx.x;
+    const double *v = V + (size_t)j * ldv;
+    double sacc = 0.0;
+    for (int i = t; i < n; i += 256) sacc = fma(v[i], w[i], sacc);
+    red[t] = sacc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) hout[j] = red[0];
+}
+
+// w[i] -= sum_{j<k} V[i + j ldv] h[j]
+__global__ void sub_vh_kernel(double *__restrict__ w, const double *__restrict__ V, int ldv, int k,
+                              const double *__restrict__ hv, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sacc = 0.0;
+    for (int j = 0; j < k; ++j) sacc = fma(V[i + (size_t)j * ldv], hv[j], sacc);
+    w[i] -= sacc;
+}
+
+// o[i] = w[i] / sqrt(*nrm2)
+__global__ void scale_by_norm_kernel(const double *__restrict__ w, const double *__restrict__ nrm2, int n,
+                                     double *__restrict__ o)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = w[i] / sqrt(*nrm2);
+}
+
+// w = sum_cs part[cs] (the second pass of the deterministic GEMV)
+__global__ void gemv_combine_kernel(const double *__restrict__ part, int n, double *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sacc = 0.0;
+    for (int cs = 0; cs < GEMV_CS; ++cs) sacc += part[(size_t)cs * n + i];
+    w[i] = sacc;
+}
+
+// x[i] += sum_{j<k} Z[i + j ldz] y[j], with max |x_i|
+__global__ void xupdate_kernel(double *__restrict__ x, const double *__restrict__ Z, int ldz, int k,
+                               const double *__restrict__ y, int n, unsigned long long *xmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sacc = 0.0;
+    for (int j = 0; j < k; ++j) sacc = fma(Z[i + (size_t)j * ldz], y[j], sacc);
+    const double xi = x[i] + sacc;
+    x[i] = xi;
+    atomicMax(xmax, (unsigned long long)__double_as_longlong(fabs(xi)));
+}
+
+template <typename T>
+__global__ void widen_kernel(const T *__restrict__ d, int n, double *__restrict__ z)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = (double)d[i];
+}
+
+// z = V y (FP64), the butterfly applied to a preconditioner output
+template <typename T>
+__global__ void rbt_right_kernel(const T *__restrict__ y, const double *__restrict__ PV, int n, double *__restrict__ z)
+{
+    const int q = n / 4;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= q) return;
+    double M[4][4];
+    rbt_group(PV, n, g, M);
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (double)y[g + k * q];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double sacc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sacc = fma(M[r][k], v[k], sacc);
+        z[g + r * q] = sacc;
+    }
+}
+
 #define IR_TRY(x)                                                       \
     do {                                                                \
         hipError_t e__ = (x);                                           \
@@ -310,9 +530,135 @@ int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const
     return NROCM_OK;
 }
 
+// Flexible GMRES(GM_RESTART) in FP64 on A x = b, right-preconditioned by the
+// working-precision factors (M^{-1} v = V_rbt U^{-1} L^{-1} U_rbt^T v, or with
+// the row permutation instead of the butterflies): GMRES-based iterative
+// refinement (Carson & Higham 2017).  Classical Gram-Schmidt applied twice;
+// the Hessenberg matrix and its Givens rotations live on the host (one small
+// copy per iteration).  After every cycle x is updated and the true residual
+// decides convergence with the same normwise backward error as classic
+// refinement; `iters` counts preconditioner solves.
+template <typename T>
+int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, const T *W, T *d,
+           const int *perm, const double *PU, const double *PV, const T *InvL, const T *InvU, double *r,
+           double *part, int *sync, int *err, unsigned long long *red, const int *info, double *Vk, double *Zk,
+           double *hdev, bool rbt, double tol, int maxit, int *iters, double *berr_out)
+{
+    hipStream_t st = h->stream;
+    constexpr int M = GM_RESTART;
+    double *h1 = hdev, *h2 = hdev + (M + 2), *hn = hdev + 2 * (M + 2), *ydev = hdev + 3 * (M + 2);
+    const unsigned nb = nrocm_cdiv(n, 256);
+    int solves = 0;
+    double berr = INFINITY;
+    double nrmA = 0.0, nrmB = 0.0;
+    int s = NROCM_OK;
+    // H column-major (M+1) x M on the host, Givens (c, s), g
+    std::vector<double> H((size_t)(M + 1) * M), cs(M), sn(M), g(M + 1), hbuf(3 * (M + 2)), y(M);
+    for (;;) {
+        // true residual r = b - A x and the backward error (x max recomputed)
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
+        hipLaunchKernelGGL(absmax_kernel, dim3(nb), dim3(256), 0, st, x, n, red + 3);
+        hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda, n, x,
+                           part);
+        hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, st, b, part, n, r, red + 2);
+        hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(256), 0, st, r, n, r, n, hn);
+        IR_TRY(hipGetLastError());
+        unsigned long long hred[4];
+        int hinfo[2];
+        double beta2 = 0.0;
+        IR_TRY(hipMemcpyAsync(hred, red, sizeof(hred), hipMemcpyDeviceToHost, st));
+        IR_TRY(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+        IR_TRY(hipMemcpyAsync(&beta2, hn, sizeof(double), hipMemcpyDeviceToHost, st));
+        IR_TRY(hipStreamSynchronize(st));
+        if (hinfo[1]) return nrocm_fail(h, NROCM_ERR_HIP, "triangular solve wait timed out");
+        double v[4];
+        for (int q = 0; q < 4; ++q) {
+            long long bits = (long long)hred[q];
+            std::memcpy(&v[q], &bits, sizeof(double));
+        }
+        nrmA = v[0];
+        nrmB = v[1];
+        berr = v[2] / (nrmA * v[3] + nrmB);
+        if (!(berr > tol) || solves >= maxit || hinfo[0] != 0) break;
+        const double beta = std::sqrt(beta2);
+        if (!(beta > 0.0)) break;
+        // V_0 = r / beta
+        hipLaunchKernelGGL(scale_by_norm_kernel, dim3(nb), dim3(256), 0, st, r, hn, n, Vk);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        // stop a cycle once the estimated |r|_2 (>= |r|_inf) meets the bound
+        const double target = tol * (nrmA * v[3] + nrmB);
+        int k = 0;
+        for (int j = 0; j < M && solves < maxit; ++j) {
+            double *vj = Vk + (size_t)j * n, *zj = Zk + (size_t)j * n, *w = Vk + (size_t)(j + 1) * n;
+            // z_j = M^{-1} v_j
+            if (rbt)
+                hipLaunchKernelGGL((rbt_left_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, vj, PU, n, d);
+            else
+                hipLaunchKernelGGL((gather_kernel<T>), dim3(nb), dim3(256), 0, st, vj, perm, d, n);
+            IR_TRY(hipGetLastError());
+            s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
+            if (s != NROCM_OK) return s;
+            if (rbt)
+                hipLaunchKernelGGL((rbt_right_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, d, PV, n, zj);
+            else
+                hipLaunchKernelGGL((widen_kernel<T>), dim3(nb), dim3(256), 0, st, d, n, zj);
+            ++solves;
+            // w = A z_j, orthogonalised twice against v_0..v_j
+            hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda,
+                               n, zj, part);
+            hipLaunchKernelGGL(gemv_combine_kernel, dim3(nb), dim3(256), 0, st, part, n, w);
+            hipLaunchKernelGGL(dots_kernel, dim3(j + 1), dim3(256), 0, st, Vk, n, w, n, h1);
+            hipLaunchKernelGGL(sub_vh_kernel, dim3(nb), dim3(256), 0, st, w, Vk, n, j + 1, h1, n);
+            hipLaunchKernelGGL(dots_kernel, dim3(j + 1), dim3(256), 0, st, Vk, n, w, n, h2);
+            hipLaunchKernelGGL(sub_vh_kernel, dim3(nb), dim3(256), 0, st, w, Vk, n, j + 1, h2, n);
+            hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(256), 0, st, w, n, w, n, hn);
+            hipLaunchKernelGGL(scale_by_norm_kernel, dim3(nb), dim3(256), 0, st, w, hn, n, w);
+            IR_TRY(hipGetLastError());
+            IR_TRY(hipMemcpyAsync(hbuf.data(), hdev, 3 * (M + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+            IR_TRY(hipStreamSynchronize(st));
+            double *col = H.data() + (size_t)j * (M + 1);
+            for (int i = 0; i <= j; ++i) col[i] = hbuf[i] + hbuf[(M + 2) + i];
+            col[j + 1] = std::sqrt(hbuf[2 * (M + 2)]);
+            for (int i = 0; i < j; ++i) {  // previous rotations
+                const double t0 = cs[i] * col[i] + sn[i] * col[i + 1];
+                col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+                col[i] = t0;
+            }
+            const double den = std::hypot(col[j], col[j + 1]);
+            cs[j] = den > 0.0 ? col[j] / den : 1.0;
+            sn[j] = den > 0.0 ? col[j + 1] / den : 0.0;
+            col[j] = den;
+            col[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            k = j + 1;
+            if (std::fabs(g[j + 1]) <= target || !(hbuf[2 * (M + 2)] > 0.0)) break;
+        }
+        if (k == 0) break;
+        // y = H(0:k, 0:k)^{-1} g(0:k); x += Z y
+        for (int i = k - 1; i >= 0; --i) {
+            double t0 = g[i];
+            for (int c = i + 1; c < k; ++c) t0 -= H[(size_t)c * (M + 1) + i] * y[c];
+            y[i] = t0 / H[(size_t)i * (M + 1) + i];
+        }
+        IR_TRY(hipMemcpyAsync(ydev, y.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 3, 1);
+        hipLaunchKernelGGL(xupdate_kernel, dim3(nb), dim3(256), 0, st, x, Zk, n, k, ydev, n, red + 3);
+        IR_TRY(hipGetLastError());
+        IR_TRY(hipStreamSynchronize(st));  // y lives on the host stack frame
+    }
+    if (iters) *iters = solves;
+    if (berr_out) *berr_out = berr;
+    int hinfo0 = 0;
+    IR_TRY(hipMemcpy(&hinfo0, info, sizeof(int), hipMemcpyDeviceToHost));
+    if (hinfo0 != 0) return nrocm_fail(h, NROCM_ERR_NOT_CONVERGED, "singular factorization");
+    return berr <= tol ? NROCM_OK : NROCM_ERR_NOT_CONVERGED;
+}
+
 template <typename T>
 int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, NrocmTrailing trail,
-            double tol, int maxit, int *iters, double *berr_out)
+            bool rbt, bool window, bool gmres, double tol, int maxit, int *iters, double *berr_out)
 {
     hipStream_t st = h->stream;
     const int nblk = (n + BS - 1) / BS;
@@ -324,7 +670,9 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     const size_t oW = take(szW), oIpiv = take(n * sizeof(int)), oPerm = take(n * sizeof(int)),
                  oR = take(szVec), oD = take((size_t)n * sizeof(T)), oPart = take(GEMV_CS * szVec),
                  oSync = take((nblk + 2) * sizeof(int)), oRed = take(8 * sizeof(unsigned long long)),
-                 oInfo = take(2 * sizeof(int)), oInv = take(2 * (size_t)nblk * BS * BS * sizeof(T));
+                 oInfo = take(2 * sizeof(int)), oInv = take(2 * (size_t)nblk * BS * BS * sizeof(T)),
+                 oRbt = take(rbt ? 4 * szVec : 0),
+                 oKry = take(gmres ? (size_t)(2 * GM_RESTART + 1) * szVec + 4 * (GM_RESTART + 2) * sizeof(double) : 0);
     void *ws = nullptr;
     int s = nrocm_workspace(h, off, &ws);
     if (s != NROCM_OK) return s;
@@ -337,23 +685,39 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     unsigned long long *red = (unsigned long long *)(base + oRed);  // [0]=|A|, [1]=|b|, [2]=|r|, [3]=|x|
     int *info = (int *)(base + oInfo), *err = info + 1;
     T *InvL = (T *)(base + oInv), *InvU = InvL + (size_t)nblk * BS * BS;
+    double *PU = (double *)(base + oRbt), *PV = PU + 2 * (size_t)n;
+    double *Vk = (double *)(base + oKry), *Zk = Vk + (size_t)(GM_RESTART + 1) * n;  // Krylov basis, preconditioned
+    double *hdev = Zk + (size_t)GM_RESTART * n;  // h1 | h2 | norm | y, GM_RESTART + 2 each
 
     const size_t tot = (size_t)n * n;
-    if constexpr (sizeof(T) == 4)
+    if (rbt) {
+        hipLaunchKernelGGL(rbt_params_kernel, dim3(nrocm_cdiv(4 * n, 256)), dim3(256), 0, st, PU, n, 0x5EEDu);
+        IR_TRY(hipGetLastError());
+        const size_t groups = (size_t)(n / 4) * (n / 4);
+        hipLaunchKernelGGL((rbt_matrix_kernel<T>), dim3(nrocm_cdiv(groups, 256)), dim3(256), 0, st, A, lda, n, PU,
+                           PV, W);
+    } else if constexpr (sizeof(T) == 4) {
         hipLaunchKernelGGL(cvt_d2s_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, W, n, n);
-    else
+    } else {
         hipLaunchKernelGGL(copy_d_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, (double *)W, n, n);
+    }
     IR_TRY(hipGetLastError());
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, info, 2);
     IR_TRY(hipGetLastError());
     int fs;
     if constexpr (sizeof(T) == 4)
-        fs = nrocm_getrf_s(h, st, n, n, (float *)W, n, ipiv, info, trail);
+        fs = rbt      ? nrocm_getrf_nopiv_s(h, n, (float *)W, n, info, trail)
+             : window ? nrocm_getrf_window_s(h, n, (float *)W, n, ipiv, info, trail)
+                      : nrocm_getrf_s(h, st, n, n, (float *)W, n, ipiv, info, trail);
     else
-        fs = nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
+        fs = rbt      ? nrocm_getrf_nopiv_d(h, n, (double *)W, n, info, trail)
+             : window ? nrocm_getrf_window_d(h, n, (double *)W, n, ipiv, info, trail)
+                      : nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
     if (fs != NROCM_OK) return fs;
-    hipLaunchKernelGGL(perm_kernel, dim3(1), dim3(1024), n * sizeof(int), st, ipiv, n, perm);
-    IR_TRY(hipGetLastError());
+    if (!rbt) {
+        hipLaunchKernelGGL(perm_kernel, dim3(1), dim3(1024), n * sizeof(int), st, ipiv, n, perm);
+        IR_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL((inv_diag_kernel<T, true>), dim3(nblk), dim3(64), 0, st, W, n, n, InvL);
     IR_TRY(hipGetLastError());
     hipLaunchKernelGGL((inv_diag_kernel<T, false>), dim3(nblk), dim3(64), 0, st, W, n, n, InvU);
@@ -372,13 +736,22 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     int it = 0;
     unsigned long long hred[4];
     int hinfo[2];
+    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, PU, PV, InvL, InvU, r, part, sync, err, red, info, Vk,
+                             Zk, hdev, rbt, tol, maxit, iters, berr_out);
     for (it = 1; it <= maxit; ++it) {
-        hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
+        if (rbt)
+            hipLaunchKernelGGL((rbt_left_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, r, PU, n, d);
+        else
+            hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
         IR_TRY(hipGetLastError());
         s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
         if (s != NROCM_OK) return s;
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
-        hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
+        if (rbt)
+            hipLaunchKernelGGL((rbt_right_update_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, x, d, PV,
+                               n, red + 3);
+        else
+            hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
         hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda, n, x,
                            part);
         hipLaunchKernelGGL(residual_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, b, part, n, r, red + 2);
@@ -415,7 +788,11 @@ int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double
     if (lda < std::max(1, n)) return -4;
     if (!b && n > 0) return -5;
     if (!x && n > 0) return -6;
-    if (precision < NROCM_PREC_FP64 || precision > NROCM_PREC_BF16X3) return -7;
+    const bool rbt = (precision & NROCM_PREC_RBT) != 0;
+    const bool gmres = (precision & NROCM_PREC_GMRES) != 0;
+    const bool window = (precision & NROCM_PREC_WINDOW) != 0;
+    precision &= ~(NROCM_PREC_RBT | NROCM_PREC_GMRES | NROCM_PREC_WINDOW);
+    if (precision < NROCM_PREC_FP64 || precision > NROCM_PREC_BF16X3 || (rbt && window)) return -7;
     if (!(tol >= 0.0)) return -8;
     if (maxit < 1) return -9;
     if (n == 0) {
@@ -423,12 +800,14 @@ int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double
         if (berr_host) *berr_host = 0.0;
         return NROCM_OK;
     }
+    if (rbt && n % 4 != 0)
+        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "dgesv_ir: the butterfly transformation needs n % 4 == 0");
     if (precision == NROCM_PREC_FP64)
-        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, tol, maxit, iters_host, berr_host);
+        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, rbt, window, gmres, tol, maxit, iters_host, berr_host);
     const NrocmTrailing tr = precision == NROCM_PREC_BF16   ? TRAIL_BF16
                              : precision == NROCM_PREC_BF16X3 ? TRAIL_BF16X3
                                                               : TRAIL_F32;
-    return gesv_ir<float>(h, n, a, lda, b, x, tr, tol, maxit, iters_host, berr_host);
+    return gesv_ir<float>(h, n, a, lda, b, x, tr, rbt, window, gmres, tol, maxit, iters_host, berr_host);
 }
 
 }  // extern "C"

@@ -50,6 +50,100 @@ def test_mixed_ir_converges(nl, orc, n, prec):
         assert iters <= 6
 
 
+@pytest.mark.parametrize("n", [512, 1024])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_mixed_ir_butterfly_converges(nl, orc, n, prec):
+    """U^T A V (random butterflies of depth 2) factored without pivoting, then
+    FP64 refinement in the original space: same bars as the pivoted solver."""
+    A = kappa_matrix(170 + n, n)
+    b = gen.normal(180 + n, (n,))
+    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32, "bf16x3": nl.PREC_BF16X3}[prec] | nl.PREC_RBT
+    print(f"butterfly IR n={n} {prec}", flush=True)
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
+                                         tol=1e-14, maxit=30)
+    xv = x.numpy()[:, 0]
+    print(f"  iters={iters} berr={be:.3e} ok={ok}")
+    assert ok and be <= 1e-14
+    assert berr(A, xv, b) <= 2e-14
+    LU, ip, _ = orc.getrf(A)
+    xr, _ = orc.getrs(LU, ip, b)
+    fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
+    assert fe <= 1e4 * 1e-14, fe
+
+
+@pytest.mark.parametrize("n", [300, 1024])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_mixed_ir_window_converges(nl, orc, n, prec):
+    """Pivots searched in the next 128 rows of each 64-column block only: same
+    bars as the pivoted solver, and at most 8 refinement steps (the growth
+    stays close to partial pivoting's on these matrices)."""
+    A = kappa_matrix(370 + n, n)
+    b = gen.normal(380 + n, (n,))
+    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32, "bf16x3": nl.PREC_BF16X3}[prec] | nl.PREC_WINDOW
+    print(f"window IR n={n} {prec}", flush=True)
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
+                                         tol=1e-14, maxit=30)
+    xv = x.numpy()[:, 0]
+    print(f"  iters={iters} berr={be:.3e} ok={ok}")
+    assert ok and be <= 1e-14
+    assert berr(A, xv, b) <= 2e-14
+    LU, ip, _ = orc.getrf(A)
+    xr, _ = orc.getrs(LU, ip, b)
+    fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
+    assert fe <= 1e4 * 1e-14, fe
+    assert iters <= (2 if prec == "fp64" else 8)
+
+
+def test_mixed_ir_window_singular(nl):
+    """An all-zero column cannot be pivoted inside any window: reported as a
+    singular factorisation, not as a converged solve."""
+    n = 256
+    A = kappa_matrix(395, n)
+    A[:, 70] = 0.0
+    b = gen.normal(396, (n,))
+    _, _, _, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
+                                    nl.PREC_FP32 | nl.PREC_WINDOW, tol=1e-14, maxit=5)
+    assert not ok
+
+
+def test_mixed_ir_window_and_butterfly_exclusive(nl):
+    A = kappa_matrix(397, 128)
+    b = gen.normal(398, (128,))
+    with pytest.raises(nl.NrocmError):
+        nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
+                          nl.PREC_FP32 | nl.PREC_WINDOW | nl.PREC_RBT, tol=1e-14, maxit=5)
+
+
+@pytest.mark.parametrize("variant", ["fp32_rbt", "fp32_window", "bf16x3_window", "fp32"])
+def test_gmres_ir_converges(nl, orc, variant):
+    """FGMRES refinement preconditioned by the low-precision factors reaches
+    the 1e-14 bar."""
+    n = 1024
+    A = kappa_matrix(270, n)
+    b = gen.normal(280, (n,))
+    base = {"fp32_rbt": nl.PREC_FP32 | nl.PREC_RBT, "fp32_window": nl.PREC_FP32 | nl.PREC_WINDOW,
+            "bf16x3_window": nl.PREC_BF16X3 | nl.PREC_WINDOW, "fp32": nl.PREC_FP32}[variant]
+    print(f"gmres IR n={n} {variant}", flush=True)
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
+                                         base | nl.PREC_GMRES, tol=1e-14, maxit=60)
+    xv = x.numpy()[:, 0]
+    print(f"  solves={iters} berr={be:.3e} ok={ok}")
+    assert ok and be <= 1e-14
+    assert berr(A, xv, b) <= 2e-14
+    LU, ip, _ = orc.getrf(A)
+    xr, _ = orc.getrs(LU, ip, b)
+    fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
+    assert fe <= 1e4 * 1e-14, fe
+
+
+def test_mixed_ir_butterfly_needs_multiple_of_4(nl):
+    A = kappa_matrix(190, 102)
+    b = gen.normal(191, (102,))
+    with pytest.raises(nl.NrocmError):
+        nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
+                          nl.PREC_FP32 | nl.PREC_RBT, tol=1e-14, maxit=10)
+
+
 def test_mixed_ir_plain_bf16_reports(nl):
     """Plain BF16 operands at kappa = 1e4 sit outside classical IR's
     convergence region (kappa * u_bf16 ~ 39); the call must still return a
