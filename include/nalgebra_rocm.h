@@ -45,13 +45,10 @@ extern "C" {
 #define NROCM_PREC_FP32 1
 #define NROCM_PREC_BF16 2
 #define NROCM_PREC_BF16X3 3  /* BF16 MFMA on three-way split operands (~FP32 accuracy) */
-/* OR-ed onto a precision of nrocm_dgesv_ir: factor U^T A V without pivoting,
- * U and V random butterflies of depth 2, instead of using partial pivoting. */
-#define NROCM_PREC_RBT 16
 /* OR-ed onto a precision of nrocm_dgesv_ir: refine with flexible GMRES in FP64
  * preconditioned by the low-precision factors instead of classic refinement. */
 #define NROCM_PREC_GMRES 32
-/* OR-ed onto a precision of nrocm_dgesv_ir (not with NROCM_PREC_RBT): the pivot
+/* OR-ed onto a precision of nrocm_dgesv_ir: the pivot
  * of every column is searched among the next 128 rows of its 64-column block
  * only (windowed partial pivoting), instead of among all remaining rows. */
 #define NROCM_PREC_WINDOW 64
@@ -198,12 +195,7 @@ int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *co
  * or maxit solves.  A and b are not modified.  iters_host / berr_host receive
  * the solves taken and the final backward error (the call synchronises the
  * stream once per refinement step).  Returns NROCM_ERR_NOT_CONVERGED if tol
- * was not reached.  With NROCM_PREC_RBT set (n a multiple of 4), the factored
- * matrix is U^T A V for seeded random recursive butterflies U, V of depth 2
- * (Parker 1995; Baboulin, Dongarra et al. 2013) and the LU has no pivoting:
- * every step is a blocked diagonal-block factorisation, triangular solves and
- * an MFMA trailing update; each refinement step solves U^T A V y = U^T r and
- * adds V y.  With NROCM_PREC_WINDOW set, each 64-column block is factored
+ * was not reached.  With NROCM_PREC_WINDOW set, each 64-column block is factored
  * with its pivots chosen among the 128 rows starting at the block's diagonal
  * (one workgroup, no full-height pivot search); the rows below that window
  * are eliminated by a triangular solve with the block's U.  With

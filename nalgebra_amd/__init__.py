@@ -11,7 +11,7 @@ The Python layer mirrors the reference's API for this path:
 All arithmetic runs in libnalgebra_rocm.so (HIP kernels behind a C ABI);
 torch supplies device memory and the stream.  There is no CPU fallback.
 """
-from ._lib import EXACT, FAST, PREC_BF16, PREC_BF16X3, PREC_FP32, PREC_FP64, PREC_GMRES, PREC_RBT, PREC_WINDOW, NrocmError, header_functions
+from ._lib import EXACT, FAST, PREC_BF16, PREC_BF16X3, PREC_FP32, PREC_FP64, PREC_GMRES, PREC_WINDOW, NrocmError, header_functions
 from .linalg import (
     DMatrix,
     LU,
@@ -25,7 +25,7 @@ from .linalg import (
 from .batched import CholeskyBatch, LeastSquaresBatch
 
 __all__ = [
-    "EXACT", "FAST", "PREC_FP64", "PREC_FP32", "PREC_BF16", "PREC_BF16X3", "PREC_RBT", "PREC_GMRES", "PREC_WINDOW", "NrocmError", "header_functions",
+    "EXACT", "FAST", "PREC_FP64", "PREC_FP32", "PREC_BF16", "PREC_BF16X3", "PREC_GMRES", "PREC_WINDOW", "NrocmError", "header_functions",
     "DMatrix", "LU", "Cholesky", "QR", "PermutationSequence", "gemm", "handle", "solve_mixed_ir",
     "CholeskyBatch", "LeastSquaresBatch",
 ]

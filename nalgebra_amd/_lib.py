@@ -24,7 +24,6 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
 PREC_BF16X3 = 3
-PREC_RBT = 16  # OR-ed: random butterfly transformation + LU without pivoting
 PREC_GMRES = 32  # OR-ed: FGMRES-based refinement preconditioned by the factors
 PREC_WINDOW = 64  # OR-ed: pivots searched in the next 128 rows of each 64-column block
 

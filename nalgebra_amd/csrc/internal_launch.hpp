@@ -24,13 +24,14 @@ int nrocm_getrf_d(nrocm_handle h, hipStream_t st, int m, int n, double *A, int l
                   NrocmTrailing trail);
 int nrocm_getrf_s(nrocm_handle h, hipStream_t st, int m, int n, float *A, int lda, int *ipiv, int *info,
                   NrocmTrailing trail);
-// LU without pivoting (square n), for the randomized mixed-precision solver.
-int nrocm_getrf_nopiv_d(nrocm_handle h, int n, double *A, int lda, int *info, NrocmTrailing trail);
-int nrocm_getrf_nopiv_s(nrocm_handle h, int n, float *A, int lda, int *info, NrocmTrailing trail);
 // LU with the pivot search of each 64-column block restricted to the next 128
 // rows (square n), for the mixed-precision solver.
-int nrocm_getrf_window_d(nrocm_handle h, int n, double *A, int lda, int *ipiv, int *info, NrocmTrailing trail);
-int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, int *info, NrocmTrailing trail);
+// perm (optional) receives the composed row permutation: row i of the factors
+// is row perm[i] of A.
+int nrocm_getrf_window_d(nrocm_handle h, int n, double *A, int lda, int *ipiv, int *perm, int *info,
+                         NrocmTrailing trail);
+int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, int *perm, int *info,
+                         NrocmTrailing trail);
 
 // chol_batched.hip: n <= 64 single-matrix path.
 int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int ldb, int nrhs, int *info,

@@ -355,41 +355,21 @@ __global__ __launch_bounds__(256) void potrs_kernel(const double *L, int lda, in
     }
 }
 
-// Blocked right-looking LU without pivoting (64-wide panels): the diagonal
-// block in one workgroup, L21 and U12 by triangular solves, then the same
-// MFMA trailing update as getrf_impl.
-template <typename T>
-int getrf_nopiv_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int *info, NrocmTrailing trail)
-{
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, info, 1, 0);
-    LU_TRY(hipGetLastError());
-    constexpr int NB = 64;
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int nb = std::min(NB, n - k0);
-        const int k1 = k0 + nb;
-        hipLaunchKernelGGL((diag_lu_nopiv_kernel<T>), dim3(1), dim3(256), 0, st, A, lda, k0, nb, info);
-        LU_TRY(hipGetLastError());
-        if (k1 < n) {
-            hipLaunchKernelGGL((trsm_right_upper_kernel<T>), dim3(nrocm_cdiv(n - k1, 256)), dim3(256), 0, st, A, lda,
-                               k0, nb, k1, n);
-            LU_TRY(hipGetLastError());
-            LU_TRY(trsm<T>(st, A, lda, k0, nb, k1, n, false));
-            LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k1, n, k0, nb));
-        }
-    }
-    return NROCM_OK;
-}
-
 // Blocked LU with the pivot search of each 64-column block restricted to a
 // window of WINDOW_ROWS rows (window_panel_kernel), then the window's row
 // plan on the other columns, A21 U11^{-1} for the rows below the window, U12
-// and the trailing update.  ipiv has the LAPACK meaning, as for getrf_impl.
+// and the trailing update.  ipiv has the LAPACK meaning, as for getrf_impl;
+// perm (optional) receives the composed row permutation.
 template <typename T>
-int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int *ipiv, int *info,
+int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int *ipiv, int *perm, int *info,
                       NrocmTrailing trail)
 {
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, info, 1, 0);
     LU_TRY(hipGetLastError());
+    if (perm) {
+        hipLaunchKernelGGL(iota_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, perm, n);
+        LU_TRY(hipGetLastError());
+    }
     int *plan = (int *)h->small + NROCM_SMALL_PLAN;
     constexpr int NB = 64;
     for (int k0 = 0; k0 < n; k0 += NB) {
@@ -397,7 +377,7 @@ int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int 
         const int k1 = k0 + nb;
         const int kw = std::min(n, k0 + WINDOW_ROWS);
         hipLaunchKernelGGL((window_panel_kernel<T>), dim3(1), dim3(WINDOW_ROWS), 0, st, A, lda, k0, nb, kw, ipiv,
-                           info, plan);
+                           info, plan, perm);
         LU_TRY(hipGetLastError());
         const int ncol = k0 + (n - k1);
         if (ncol > 0) {
@@ -406,8 +386,8 @@ int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int 
             LU_TRY(hipGetLastError());
         }
         if (kw < n) {
-            hipLaunchKernelGGL((trsm_right_upper_kernel<T>), dim3(nrocm_cdiv(n - kw, 256)), dim3(256), 0, st, A, lda,
-                               k0, nb, kw, n);
+            hipLaunchKernelGGL((trsm_right_upper_kernel<T>), dim3(nrocm_cdiv(n - kw, TRSM_RU_THREADS)),
+                               dim3(TRSM_RU_THREADS), 0, st, A, lda, k0, nb, kw, n);
             LU_TRY(hipGetLastError());
         }
         if (k1 < n) {
@@ -450,42 +430,25 @@ int nrocm_getrf_s(nrocm_handle h, hipStream_t st, int m, int n, float *A, int ld
 }
 
 template <typename T>
-int getrf_nopiv_graph(nrocm_handle h, int n, T *A, int lda, int *info, NrocmTrailing trail)
+int getrf_window_graph(nrocm_handle h, int n, T *A, int lda, int *ipiv, int *perm, int *info, NrocmTrailing trail)
 {
-    if (n <= 64) return getrf_nopiv_impl<T>(h, h->stream, n, A, lda, info, trail);
-    const long long key[8] = {2000 + (long long)sizeof(T), n, n, lda, (long long)A, 0, (long long)info,
-                              (long long)trail};
-    return nrocm_graph_run(h, key, [&](hipStream_t st) { return getrf_nopiv_impl<T>(h, st, n, A, lda, info, trail); });
+    if (n <= 64) return getrf_window_impl<T>(h, h->stream, n, A, lda, ipiv, perm, info, trail);
+    const long long key[8] = {3000 + (long long)sizeof(T) + 16 * (long long)trail, n, lda, (long long)A,
+                              (long long)ipiv, (long long)perm, (long long)info, 0};
+    return nrocm_graph_run(
+        h, key, [&](hipStream_t st) { return getrf_window_impl<T>(h, st, n, A, lda, ipiv, perm, info, trail); });
 }
 
-template <typename T>
-int getrf_window_graph(nrocm_handle h, int n, T *A, int lda, int *ipiv, int *info, NrocmTrailing trail)
+int nrocm_getrf_window_d(nrocm_handle h, int n, double *A, int lda, int *ipiv, int *perm, int *info,
+                         NrocmTrailing trail)
 {
-    if (n <= 64) return getrf_window_impl<T>(h, h->stream, n, A, lda, ipiv, info, trail);
-    const long long key[8] = {3000 + (long long)sizeof(T), n, n, lda, (long long)A, (long long)ipiv, (long long)info,
-                              (long long)trail};
-    return nrocm_graph_run(h, key,
-                           [&](hipStream_t st) { return getrf_window_impl<T>(h, st, n, A, lda, ipiv, info, trail); });
+    return getrf_window_graph<double>(h, n, A, lda, ipiv, perm, info, trail);
 }
 
-int nrocm_getrf_window_d(nrocm_handle h, int n, double *A, int lda, int *ipiv, int *info, NrocmTrailing trail)
+int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, int *perm, int *info,
+                         NrocmTrailing trail)
 {
-    return getrf_window_graph<double>(h, n, A, lda, ipiv, info, trail);
-}
-
-int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, int *info, NrocmTrailing trail)
-{
-    return getrf_window_graph<float>(h, n, A, lda, ipiv, info, trail);
-}
-
-int nrocm_getrf_nopiv_d(nrocm_handle h, int n, double *A, int lda, int *info, NrocmTrailing trail)
-{
-    return getrf_nopiv_graph<double>(h, n, A, lda, info, trail);
-}
-
-int nrocm_getrf_nopiv_s(nrocm_handle h, int n, float *A, int lda, int *info, NrocmTrailing trail)
-{
-    return getrf_nopiv_graph<float>(h, n, A, lda, info, trail);
+    return getrf_window_graph<float>(h, n, A, lda, ipiv, perm, info, trail);
 }
 
 // ---------------------------------------------------------------- C ABI

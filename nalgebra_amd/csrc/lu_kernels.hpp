@@ -252,9 +252,16 @@ __global__ __launch_bounds__(256) void trsm_unit_lower_kernel(T *A, int lda, int
                                                               int c1, int exact)
 {
     __shared__ T L[64][65];
-    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
-        const int rr = e % k, cc = e / k;
-        L[rr][cc] = A[(r0 + rr) + (size_t)(r0 + cc) * lda];
+    {
+        // thread t stages row t & 63, columns (t >> 6) + 4 i: 16 loads in
+        // flight, clamped in-bounds addresses (entries past k are never read)
+        const int rr = threadIdx.x & 63, cq = threadIdx.x >> 6;
+        const T *lrow = A + (r0 + (rr < k ? rr : 0));
+        T v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = lrow[(size_t)(r0 + (cq + 4 * i < k ? cq + 4 * i : 0)) * lda];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) L[rr][cq + 4 * i] = v[i];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -494,68 +501,65 @@ __global__ __launch_bounds__(256) void apply_plan_kernel(T *A, int lda, const in
     if (valid && slot < nt) A[plan[1 + slot] + (size_t)c * lda] = v;
 }
 
-// ---------------------------------------------------------------- LU without pivoting
-// (for matrices made safe to factor without row exchanges, e.g. by a random
-// butterfly transformation; used only by the mixed-precision solver)
-
-// In-place LU of the nb x nb diagonal block at (k0, k0), nb <= 64, held in LDS
-// by one workgroup.  A zero pivot sets info (first one) and the step is skipped.
-template <typename T>
-__global__ __launch_bounds__(256) void diag_lu_nopiv_kernel(T *A, int lda, int k0, int nb, int *info)
-{
-    __shared__ T D[64][65];
-    const int t = threadIdx.x;
-    for (int e = t; e < nb * nb; e += blockDim.x) D[e % nb][e / nb] = A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda];
-    __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const T d = D[c][c];
-        const int rest = nb - c - 1;
-        if (d == T(0)) {
-            if (t == 0 && info && *info == 0) *info = k0 + c + 1;
-        } else {
-            const T inv = T(1) / d;
-            for (int r = c + 1 + t; r < nb; r += blockDim.x) D[r][c] *= inv;
-            lds_barrier();
-            for (int e = t; e < rest * rest; e += blockDim.x) {
-                const int r = c + 1 + e % rest, q = c + 1 + e / rest;
-                D[r][q] = fma(-D[r][c], D[c][q], D[r][q]);
-            }
-        }
-        lds_barrier();
-    }
-    for (int e = t; e < nb * nb; e += blockDim.x) A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda] = D[e % nb][e / nb];
-}
-
 // A[r0.., k0..k0+nb) <- A[r0.., k0..k0+nb) U11^{-1} (U11 the upper triangle of
-// the factored diagonal block): one thread per row, the row in registers.
+// the factored diagonal block): one thread per row, the row in registers,
+// x_q = (a_q - sum_{p<q} x_p U_pq) / U_qq with four partial sums.  U is staged
+// transposed in LDS so that a column is contiguous; each column is read in two
+// halves of 32 whose reads are all issued before the first use (one LDS round
+// trip per half instead of one per pair of values).
+constexpr int TRSM_RU_THREADS = 64;
+
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_right_upper_kernel(T *A, int lda, int k0, int nb, int r0, int m)
+__global__ __launch_bounds__(TRSM_RU_THREADS) void trsm_right_upper_kernel(T *A, int lda, int k0, int nb, int r0,
+                                                                           int m)
 {
-    __shared__ T U[64][65];
-    __shared__ T dinv[64];
+    __shared__ __align__(16) T Ut[64][64];  // Ut[q][p] = U[p][q]
     const int t = threadIdx.x;
-    for (int e = t; e < nb * nb; e += blockDim.x) U[e % nb][e / nb] = A[(k0 + e % nb) + (size_t)(k0 + e / nb) * lda];
+    // thread t stages row t of U; 16 loads in flight (clamped in-bounds
+    // addresses, zeros selected afterwards) instead of one load per wait
+    const T *urow = A + (k0 + (t < nb ? t : 0));
+#pragma unroll
+    for (int q0 = 0; q0 < 64; q0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = urow[(size_t)(k0 + (q0 + j < nb ? q0 + j : 0)) * lda];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) Ut[q0 + j][t] = (t <= q0 + j && q0 + j < nb) ? v[j] : T(0);
+    }
     __syncthreads();
-    if (t < nb) dinv[t] = U[t][t] != T(0) ? T(1) / U[t][t] : T(0);
-    __syncthreads();
-    const int r = r0 + blockIdx.x * blockDim.x + t;
-    if (r >= m) return;
+    const int r = r0 + blockIdx.x * TRSM_RU_THREADS + t;
+    const bool live = r < m;
+    T *row = A + (live ? r : r0);
     T x[64];
 #pragma unroll
-    for (int q = 0; q < 64; ++q) x[q] = q < nb ? A[r + (size_t)(k0 + q) * lda] : T(0);
+    for (int q = 0; q < 64; ++q) x[q] = (live && q < nb) ? row[(size_t)(k0 + q) * lda] : T(0);
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-        if (q < nb) {
-            T v = x[q];
+        if (q < nb) {  // uniform
+            T s[4] = {x[q], T(0), T(0), T(0)};
 #pragma unroll
-            for (int p = 0; p < 64; ++p)
-                if (p < q) v = fma(-x[p], U[p][q], v);
-            x[q] = v * dinv[q];
+            for (int h = 0; h < 2; ++h) {
+                if (32 * h < q) {
+                    T u[32];
+#pragma unroll
+                    for (int j = 0; j < 32; ++j)
+                        if (32 * h + j < q) u[j] = Ut[q][32 * h + j];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < 32; ++j)
+                        if (32 * h + j < q) s[j & 3] = fma(-x[32 * h + j], u[j], s[j & 3]);
+                }
+            }
+            x[q] = ((s[0] + s[1]) + (s[2] + s[3])) / Ut[q][q];
         }
     }
+    int nbs = nb;
+    asm volatile("" : "+s"(nbs));  // recompute the column masks instead of keeping 64 alive
+    if (live) {
 #pragma unroll
-    for (int q = 0; q < 64; ++q)
-        if (q < nb) A[r + (size_t)(k0 + q) * lda] = x[q];
+        for (int q = 0; q < 64; ++q)
+            if (q < nbs) row[(size_t)(k0 + q) * lda] = x[q];
+    }
 }
 
 // ---------------------------------------------------------------- LU with windowed pivoting
@@ -575,11 +579,12 @@ constexpr int WINDOW_ROWS = 128;
 // its row, only the pivot row passes through LDS, and every row is written to
 // its final position at the end.  Records ipiv (absolute, LAPACK order: step
 // k0+c exchanged rows k0+c and ipiv[k0+c]) and the composed row plan of the
-// window for apply_plan_kernel.  A zero pivot column sets info (first one)
-// and is skipped.
+// window for apply_plan_kernel; with perm non-null it also applies the
+// window's permutation to perm[k0, kw) (row i of the factors is row perm[i]
+// of the input).  A zero pivot column sets info (first one) and is skipped.
 template <typename T>
 __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda, int k0, int nb, int kw, int *ipiv,
-                                                                   int *info, int *plan)
+                                                                   int *info, int *plan, int *perm)
 {
     __shared__ T prow[2][64];
     __shared__ T wv[2][2];
@@ -592,6 +597,7 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
     const T *row = A + (k0 + (live ? t : 0));
 #pragma unroll
     for (int q = 0; q < 64; ++q) a[q] = (live && q < nb) ? row[(size_t)(k0 + q) * lda] : T(0);
+    const int myperm = (perm && live) ? perm[k0 + t] : 0;
     int pos = t;
 #pragma unroll
     for (int c = 0; c < 64; ++c) {
@@ -634,6 +640,7 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
             }
         }
     }
+    if (perm && live) perm[k0 + pos] = myperm;
     if (live) {
         T *dst = A + (k0 + pos);
 #pragma unroll
@@ -651,6 +658,12 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
         plan[1 + PLAN_MAX + before] = k0 + t;
     }
     if (t == 0) plan[0] = __popcll(wmask[0]) + __popcll(wmask[1]);
+}
+
+__global__ void iota_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 
 __global__ void fill_int_kernel(int *p, int n, int v)

@@ -1,9 +1,9 @@
 // Mixed-precision LU + FP64 iterative refinement (nrocm_dgesv_ir).
 //
-//   A32 = fl32(A)                      working copy (FP64 kept for residuals);
-//                                      with NROCM_PREC_RBT: fl32(U^T A V), U, V
-//                                      random butterflies, factored unpivoted
-//   P A32 = L U                        blocked LU, trailing updates on FP32 or
+//   A32 = fl32(A)                      working copy (FP64 kept for residuals)
+//   P A32 = L U                        blocked LU (partial pivoting, or with
+//                                      NROCM_PREC_WINDOW pivots searched in the
+//                                      next 128 rows), trailing updates on FP32 or
 //                                      BF16 MFMA (FP32 accumulate); FP64 mode
 //                                      factors a copy of A with FP64 MFMA
 //   x = 0, r = b
@@ -91,9 +91,19 @@ __global__ __launch_bounds__(64) void inv_diag_kernel(const T *__restrict__ M, i
     const int r0 = R * BS;
     const int rows = min(BS, n - r0);
     const int c = threadIdx.x;
-    for (int e = c; e < BS * BS; e += 64) {
-        const int a = e % BS, b = e / BS;
-        D[a][b] = (a < rows && b < rows) ? M[(r0 + a) + (size_t)(r0 + b) * lda] : (a == b ? T(1) : T(0));
+    {
+        // lane c stages row c: loads in flight 16 at a time (clamped in-bounds
+        // addresses, identity padding selected afterwards)
+        const T *mrow = M + (r0 + (c < rows ? c : 0));
+#pragma unroll
+        for (int b0 = 0; b0 < BS; b0 += 16) {
+            T v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = mrow[(size_t)(r0 + (b0 + j < rows ? b0 + j : 0)) * lda];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                D[c][b0 + j] = (c < rows && b0 + j < rows) ? v[j] : (c == b0 + j ? T(1) : T(0));
+        }
     }
     __syncthreads();
     T x[BS];
@@ -288,132 +298,6 @@ __global__ void zero_d_kernel(double *p, int n)
     if (i < n) p[i] = 0.0;
 }
 
-// ---------------------------------------------------------------- random butterflies
-// A recursive butterfly of depth 2 on n = 4q entries mixes each group
-// {g, g+q, g+2q, g+3q} (g < q) by a 4 x 4 matrix: level 1 pairs (g, g+2q) and
-// (g+q, g+3q) with diagonals R1, S1 (n/2 each), level 2 pairs (g, g+q) and
-// (g+2q, g+3q) with R2a, S2a, R2b, S2b (n/4 each), each level (1/sqrt 2) [R S; R -S].
-// Entries are exp(rho/10), rho uniform in [-1/2, 1/2) (Baboulin et al. 2013).
-// Parameter block of one butterfly: 2n doubles; U and V take 4n.
-
-__device__ __forceinline__ double rbt_hash01(unsigned a, unsigned b)
-{
-    unsigned x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
-    x ^= x >> 15;
-    x *= 0x2C1B3C6Du;
-    x ^= x >> 12;
-    x *= 0x297A2D39u;
-    x ^= x >> 15;
-    return (double)x * 0x1p-32;  // [0, 1)
-}
-
-__global__ void rbt_params_kernel(double *P, int n, unsigned seed)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 4 * n) P[i] = exp((rbt_hash01(seed, (unsigned)i) - 0.5) / 10.0);
-}
-
-// M (row-major 4 x 4): the butterfly restricted to group g.
-__device__ __forceinline__ void rbt_group(const double *Wp, int n, int g, double (&M)[4][4])
-{
-    const int h2 = n / 2, h4 = n / 4;
-    const double *R1 = Wp, *S1 = Wp + h2, *R2a = Wp + n, *S2a = R2a + h4, *R2b = S2a + h4, *S2b = R2b + h4;
-    const double a = R1[g], b = S1[g], e = R1[g + h4], f = S1[g + h4];
-    const double p = R2a[g], q = S2a[g], u = R2b[g], v = S2b[g];
-    const double M0[4][4] = {{p * a, q * e, p * b, q * f},
-                             {p * a, -q * e, p * b, -q * f},
-                             {u * a, v * e, -u * b, -v * f},
-                             {u * a, -v * e, -u * b, v * f}};
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) M[r][c] = 0.5 * M0[r][c];
-}
-
-// W (working precision, ld n) = U^T A V, one thread per (row group, column group).
-template <typename T>
-__global__ void rbt_matrix_kernel(const double *__restrict__ A, int lda, int n, const double *__restrict__ PU,
-                                  const double *__restrict__ PV, T *__restrict__ W)
-{
-    const int q = n / 4;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)q * q) return;
-    const int gi = (int)(idx % q), gj = (int)(idx / q);
-    double MU[4][4], MV[4][4], X[4][4], Y[4][4];
-    rbt_group(PU, n, gi, MU);
-    rbt_group(PV, n, gj, MV);
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) X[r][c] = A[(gi + r * q) + (size_t)(gj + c * q) * lda];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)  // Y = X MV
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) s = fma(X[r][k], MV[k][c], s);
-            Y[r][c] = s;
-        }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)  // MU^T Y
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) s = fma(MU[k][r], Y[k][c], s);
-            W[(gi + r * q) + (size_t)(gj + c * q) * n] = (T)s;
-        }
-}
-
-// o = U^T r (rounded to the working precision)
-template <typename T>
-__global__ void rbt_left_kernel(const double *__restrict__ r, const double *__restrict__ PU, int n, T *__restrict__ o)
-{
-    const int q = n / 4;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= q) return;
-    double M[4][4];
-    rbt_group(PU, n, g, M);
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = r[g + k * q];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s = fma(M[k][c], v[k], s);
-        o[g + c * q] = (T)s;
-    }
-}
-
-// x += V y (FP64), with max |x_i| by ordered-bits atomicMax.
-template <typename T>
-__global__ void rbt_right_update_kernel(double *__restrict__ x, const T *__restrict__ y, const double *__restrict__ PV,
-                                        int n, unsigned long long *xmax)
-{
-    const int q = n / 4;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= q) return;
-    double M[4][4];
-    rbt_group(PV, n, g, M);
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (double)y[g + k * q];
-    unsigned long long mx = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s = fma(M[r][k], v[k], s);
-        const double xi = x[g + r * q] + s;
-        x[g + r * q] = xi;
-        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(xi));
-        mx = b > mx ? b : mx;
-    }
-    atomicMax(xmax, mx);
-}
-
 // ---------------------------------------------------------------- FGMRES helpers (FP64)
 
 // h[j] = sum_i V[i + j ldv] w[i], j < k: one workgroup per j, fixed-order
@@ -484,27 +368,6 @@ __global__ void widen_kernel(const T *__restrict__ d, int n, double *__restrict_
     if (i < n) z[i] = (double)d[i];
 }
 
-// z = V y (FP64), the butterfly applied to a preconditioner output
-template <typename T>
-__global__ void rbt_right_kernel(const T *__restrict__ y, const double *__restrict__ PV, int n, double *__restrict__ z)
-{
-    const int q = n / 4;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= q) return;
-    double M[4][4];
-    rbt_group(PV, n, g, M);
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (double)y[g + k * q];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double sacc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sacc = fma(M[r][k], v[k], sacc);
-        z[g + r * q] = sacc;
-    }
-}
-
 #define IR_TRY(x)                                                       \
     do {                                                                \
         hipError_t e__ = (x);                                           \
@@ -531,8 +394,7 @@ int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const
 }
 
 // Flexible GMRES(GM_RESTART) in FP64 on A x = b, right-preconditioned by the
-// working-precision factors (M^{-1} v = V_rbt U^{-1} L^{-1} U_rbt^T v, or with
-// the row permutation instead of the butterflies): GMRES-based iterative
+// working-precision factors (M^{-1} v = U^{-1} L^{-1} P v): GMRES-based iterative
 // refinement (Carson & Higham 2017).  Classical Gram-Schmidt applied twice;
 // the Hessenberg matrix and its Givens rotations live on the host (one small
 // copy per iteration).  After every cycle x is updated and the true residual
@@ -540,9 +402,9 @@ int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const
 // refinement; `iters` counts preconditioner solves.
 template <typename T>
 int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, const T *W, T *d,
-           const int *perm, const double *PU, const double *PV, const T *InvL, const T *InvU, double *r,
+           const int *perm, const T *InvL, const T *InvU, double *r,
            double *part, int *sync, int *err, unsigned long long *red, const int *info, double *Vk, double *Zk,
-           double *hdev, bool rbt, double tol, int maxit, int *iters, double *berr_out)
+           double *hdev, double tol, int maxit, int *iters, double *berr_out)
 {
     hipStream_t st = h->stream;
     constexpr int M = GM_RESTART;
@@ -592,17 +454,11 @@ int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, dou
         for (int j = 0; j < M && solves < maxit; ++j) {
             double *vj = Vk + (size_t)j * n, *zj = Zk + (size_t)j * n, *w = Vk + (size_t)(j + 1) * n;
             // z_j = M^{-1} v_j
-            if (rbt)
-                hipLaunchKernelGGL((rbt_left_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, vj, PU, n, d);
-            else
-                hipLaunchKernelGGL((gather_kernel<T>), dim3(nb), dim3(256), 0, st, vj, perm, d, n);
+            hipLaunchKernelGGL((gather_kernel<T>), dim3(nb), dim3(256), 0, st, vj, perm, d, n);
             IR_TRY(hipGetLastError());
             s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
             if (s != NROCM_OK) return s;
-            if (rbt)
-                hipLaunchKernelGGL((rbt_right_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, d, PV, n, zj);
-            else
-                hipLaunchKernelGGL((widen_kernel<T>), dim3(nb), dim3(256), 0, st, d, n, zj);
+            hipLaunchKernelGGL((widen_kernel<T>), dim3(nb), dim3(256), 0, st, d, n, zj);
             ++solves;
             // w = A z_j, orthogonalised twice against v_0..v_j
             hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda,
@@ -658,7 +514,7 @@ int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, dou
 
 template <typename T>
 int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, NrocmTrailing trail,
-            bool rbt, bool window, bool gmres, double tol, int maxit, int *iters, double *berr_out)
+            bool window, bool gmres, double tol, int maxit, int *iters, double *berr_out)
 {
     hipStream_t st = h->stream;
     const int nblk = (n + BS - 1) / BS;
@@ -671,7 +527,6 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
                  oR = take(szVec), oD = take((size_t)n * sizeof(T)), oPart = take(GEMV_CS * szVec),
                  oSync = take((nblk + 2) * sizeof(int)), oRed = take(8 * sizeof(unsigned long long)),
                  oInfo = take(2 * sizeof(int)), oInv = take(2 * (size_t)nblk * BS * BS * sizeof(T)),
-                 oRbt = take(rbt ? 4 * szVec : 0),
                  oKry = take(gmres ? (size_t)(2 * GM_RESTART + 1) * szVec + 4 * (GM_RESTART + 2) * sizeof(double) : 0);
     void *ws = nullptr;
     int s = nrocm_workspace(h, off, &ws);
@@ -685,18 +540,11 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     unsigned long long *red = (unsigned long long *)(base + oRed);  // [0]=|A|, [1]=|b|, [2]=|r|, [3]=|x|
     int *info = (int *)(base + oInfo), *err = info + 1;
     T *InvL = (T *)(base + oInv), *InvU = InvL + (size_t)nblk * BS * BS;
-    double *PU = (double *)(base + oRbt), *PV = PU + 2 * (size_t)n;
     double *Vk = (double *)(base + oKry), *Zk = Vk + (size_t)(GM_RESTART + 1) * n;  // Krylov basis, preconditioned
     double *hdev = Zk + (size_t)GM_RESTART * n;  // h1 | h2 | norm | y, GM_RESTART + 2 each
 
     const size_t tot = (size_t)n * n;
-    if (rbt) {
-        hipLaunchKernelGGL(rbt_params_kernel, dim3(nrocm_cdiv(4 * n, 256)), dim3(256), 0, st, PU, n, 0x5EEDu);
-        IR_TRY(hipGetLastError());
-        const size_t groups = (size_t)(n / 4) * (n / 4);
-        hipLaunchKernelGGL((rbt_matrix_kernel<T>), dim3(nrocm_cdiv(groups, 256)), dim3(256), 0, st, A, lda, n, PU,
-                           PV, W);
-    } else if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4) {
         hipLaunchKernelGGL(cvt_d2s_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, W, n, n);
     } else {
         hipLaunchKernelGGL(copy_d_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, st, A, lda, (double *)W, n, n);
@@ -706,15 +554,13 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     IR_TRY(hipGetLastError());
     int fs;
     if constexpr (sizeof(T) == 4)
-        fs = rbt      ? nrocm_getrf_nopiv_s(h, n, (float *)W, n, info, trail)
-             : window ? nrocm_getrf_window_s(h, n, (float *)W, n, ipiv, info, trail)
-                      : nrocm_getrf_s(h, st, n, n, (float *)W, n, ipiv, info, trail);
+        fs = window ? nrocm_getrf_window_s(h, n, (float *)W, n, ipiv, perm, info, trail)
+                    : nrocm_getrf_s(h, st, n, n, (float *)W, n, ipiv, info, trail);
     else
-        fs = rbt      ? nrocm_getrf_nopiv_d(h, n, (double *)W, n, info, trail)
-             : window ? nrocm_getrf_window_d(h, n, (double *)W, n, ipiv, info, trail)
-                      : nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
+        fs = window ? nrocm_getrf_window_d(h, n, (double *)W, n, ipiv, perm, info, trail)
+                    : nrocm_getrf_d(h, st, n, n, (double *)W, n, ipiv, info, trail);
     if (fs != NROCM_OK) return fs;
-    if (!rbt) {
+    if (!window) {
         hipLaunchKernelGGL(perm_kernel, dim3(1), dim3(1024), n * sizeof(int), st, ipiv, n, perm);
         IR_TRY(hipGetLastError());
     }
@@ -736,22 +582,15 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     int it = 0;
     unsigned long long hred[4];
     int hinfo[2];
-    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, PU, PV, InvL, InvU, r, part, sync, err, red, info, Vk,
-                             Zk, hdev, rbt, tol, maxit, iters, berr_out);
+    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, InvL, InvU, r, part, sync, err, red, info, Vk, Zk, hdev,
+                             tol, maxit, iters, berr_out);
     for (it = 1; it <= maxit; ++it) {
-        if (rbt)
-            hipLaunchKernelGGL((rbt_left_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, r, PU, n, d);
-        else
-            hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
+        hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
         IR_TRY(hipGetLastError());
         s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
         if (s != NROCM_OK) return s;
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
-        if (rbt)
-            hipLaunchKernelGGL((rbt_right_update_kernel<T>), dim3(nrocm_cdiv(n / 4, 256)), dim3(256), 0, st, x, d, PV,
-                               n, red + 3);
-        else
-            hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
+        hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);
         hipLaunchKernelGGL((gemv_part_kernel<false>), dim3(nrocm_cdiv(n, 64), GEMV_CS), dim3(256), 0, st, A, lda, n, x,
                            part);
         hipLaunchKernelGGL(residual_kernel, dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, b, part, n, r, red + 2);
@@ -788,11 +627,10 @@ int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double
     if (lda < std::max(1, n)) return -4;
     if (!b && n > 0) return -5;
     if (!x && n > 0) return -6;
-    const bool rbt = (precision & NROCM_PREC_RBT) != 0;
     const bool gmres = (precision & NROCM_PREC_GMRES) != 0;
     const bool window = (precision & NROCM_PREC_WINDOW) != 0;
-    precision &= ~(NROCM_PREC_RBT | NROCM_PREC_GMRES | NROCM_PREC_WINDOW);
-    if (precision < NROCM_PREC_FP64 || precision > NROCM_PREC_BF16X3 || (rbt && window)) return -7;
+    precision &= ~(NROCM_PREC_GMRES | NROCM_PREC_WINDOW);
+    if (precision < NROCM_PREC_FP64 || precision > NROCM_PREC_BF16X3) return -7;
     if (!(tol >= 0.0)) return -8;
     if (maxit < 1) return -9;
     if (n == 0) {
@@ -800,14 +638,12 @@ int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double
         if (berr_host) *berr_host = 0.0;
         return NROCM_OK;
     }
-    if (rbt && n % 4 != 0)
-        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "dgesv_ir: the butterfly transformation needs n % 4 == 0");
     if (precision == NROCM_PREC_FP64)
-        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, rbt, window, gmres, tol, maxit, iters_host, berr_host);
+        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, window, gmres, tol, maxit, iters_host, berr_host);
     const NrocmTrailing tr = precision == NROCM_PREC_BF16   ? TRAIL_BF16
                              : precision == NROCM_PREC_BF16X3 ? TRAIL_BF16X3
                                                               : TRAIL_F32;
-    return gesv_ir<float>(h, n, a, lda, b, x, tr, rbt, window, gmres, tol, maxit, iters_host, berr_host);
+    return gesv_ir<float>(h, n, a, lda, b, x, tr, window, gmres, tol, maxit, iters_host, berr_host);
 }
 
 }  // extern "C"

@@ -50,27 +50,6 @@ def test_mixed_ir_converges(nl, orc, n, prec):
         assert iters <= 6
 
 
-@pytest.mark.parametrize("n", [512, 1024])
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
-def test_mixed_ir_butterfly_converges(nl, orc, n, prec):
-    """U^T A V (random butterflies of depth 2) factored without pivoting, then
-    FP64 refinement in the original space: same bars as the pivoted solver."""
-    A = kappa_matrix(170 + n, n)
-    b = gen.normal(180 + n, (n,))
-    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32, "bf16x3": nl.PREC_BF16X3}[prec] | nl.PREC_RBT
-    print(f"butterfly IR n={n} {prec}", flush=True)
-    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
-                                         tol=1e-14, maxit=30)
-    xv = x.numpy()[:, 0]
-    print(f"  iters={iters} berr={be:.3e} ok={ok}")
-    assert ok and be <= 1e-14
-    assert berr(A, xv, b) <= 2e-14
-    LU, ip, _ = orc.getrf(A)
-    xr, _ = orc.getrs(LU, ip, b)
-    fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
-    assert fe <= 1e4 * 1e-14, fe
-
-
 @pytest.mark.parametrize("n", [300, 1024])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_mixed_ir_window_converges(nl, orc, n, prec):
@@ -106,22 +85,14 @@ def test_mixed_ir_window_singular(nl):
     assert not ok
 
 
-def test_mixed_ir_window_and_butterfly_exclusive(nl):
-    A = kappa_matrix(397, 128)
-    b = gen.normal(398, (128,))
-    with pytest.raises(nl.NrocmError):
-        nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
-                          nl.PREC_FP32 | nl.PREC_WINDOW | nl.PREC_RBT, tol=1e-14, maxit=5)
-
-
-@pytest.mark.parametrize("variant", ["fp32_rbt", "fp32_window", "bf16x3_window", "fp32"])
+@pytest.mark.parametrize("variant", ["fp32_window", "bf16x3_window", "fp32"])
 def test_gmres_ir_converges(nl, orc, variant):
     """FGMRES refinement preconditioned by the low-precision factors reaches
     the 1e-14 bar."""
     n = 1024
     A = kappa_matrix(270, n)
     b = gen.normal(280, (n,))
-    base = {"fp32_rbt": nl.PREC_FP32 | nl.PREC_RBT, "fp32_window": nl.PREC_FP32 | nl.PREC_WINDOW,
+    base = {"fp32_window": nl.PREC_FP32 | nl.PREC_WINDOW,
             "bf16x3_window": nl.PREC_BF16X3 | nl.PREC_WINDOW, "fp32": nl.PREC_FP32}[variant]
     print(f"gmres IR n={n} {variant}", flush=True)
     x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
@@ -134,14 +105,6 @@ def test_gmres_ir_converges(nl, orc, variant):
     xr, _ = orc.getrs(LU, ip, b)
     fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
     assert fe <= 1e4 * 1e-14, fe
-
-
-def test_mixed_ir_butterfly_needs_multiple_of_4(nl):
-    A = kappa_matrix(190, 102)
-    b = gen.normal(191, (102,))
-    with pytest.raises(nl.NrocmError):
-        nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
-                          nl.PREC_FP32 | nl.PREC_RBT, tol=1e-14, maxit=10)
 
 
 def test_mixed_ir_plain_bf16_reports(nl):
