@@ -20,18 +20,20 @@ hipError_t nrocm_gemm_s(hipStream_t st, int m, int n, int k, float alpha, const 
     return gemm<float, MODE_F32>(st, m, n, k, alpha, A, rsa, csa, B, rsb, csb, beta, C, rsc, csc);
 }
 
-hipError_t nrocm_gemm_s_bf16(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
-                             long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
-                             long csc)
+hipError_t nrocm_rank_update_d(hipStream_t st, int m, int n, int k, const double *A, long lda, const double *B,
+                               long ldb, double *C, long ldc)
 {
-    return gemm<float, MODE_BF16>(st, m, n, k, alpha, A, rsa, csa, B, rsb, csb, beta, C, rsc, csc);
+    return rank_update<double, MODE_F64>(st, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
-hipError_t nrocm_gemm_s_bf16x3(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
-                               long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
-                               long csc)
+hipError_t nrocm_rank_update_s(hipStream_t st, int m, int n, int k, const float *A, long lda, const float *B,
+                               long ldb, float *C, long ldc, int mode)
 {
-    return gemm<float, MODE_BF16X3>(st, m, n, k, alpha, A, rsa, csa, B, rsb, csb, beta, C, rsc, csc);
+    switch (mode) {
+    case MODE_BF16: return rank_update<float, MODE_BF16>(st, m, n, k, A, lda, B, ldb, C, ldc);
+    case MODE_BF16X3: return rank_update<float, MODE_BF16X3>(st, m, n, k, A, lda, B, ldb, C, ldc);
+    default: return rank_update<float, MODE_F32>(st, m, n, k, A, lda, B, ldb, C, ldc);
+    }
 }
 
 namespace {

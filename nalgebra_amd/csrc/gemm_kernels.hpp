@@ -54,7 +54,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, TS alpha
                                                    long csb, TS beta, TS *__restrict__ C, long rsc, long csc)
 {
     constexpr int BM = 64, BN = 64;
-    constexpr int BK = (MODE == MODE_BF16 || MODE == MODE_BF16X3) ? 32 : 16;
+    static_assert(MODE == MODE_F64 || MODE == MODE_F32, "general GEMM: FP64 or FP32 MFMA");
+    constexpr int BK = 16;
     using AccT = typename Acc<MODE>::T;
     using AccS = typename Acc<MODE>::S;
     __shared__ TS As[BK][BM + 1];
@@ -90,52 +91,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, TS alpha
             Bs[kk][j] = (gj < n && gk < k) ? B[(long)gk * rsb + (long)gj * csb] : TS(0);
         }
         __syncthreads();
-        if constexpr (MODE == MODE_BF16X3) {
-            s8 ah[2], am[2], al[2], bh[2], bm_[2], bl[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    float x = (float)As[lk * 8 + q][wm + t * 16 + li];
-                    short h0 = to_bf16_bits(x);
-                    float r1 = x - bf16_bits_to_float(h0);
-                    short h1 = to_bf16_bits(r1);
-                    short h2 = to_bf16_bits(r1 - bf16_bits_to_float(h1));
-                    ah[t][q] = h0; am[t][q] = h1; al[t][q] = h2;
-                    float y = (float)Bs[lk * 8 + q][wn + t * 16 + li];
-                    short g0 = to_bf16_bits(y);
-                    float s1 = y - bf16_bits_to_float(g0);
-                    short g1 = to_bf16_bits(s1);
-                    short g2 = to_bf16_bits(s1 - bf16_bits_to_float(g1));
-                    bh[t][q] = g0; bm_[t][q] = g1; bl[t][q] = g2;
-                }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    // smallest terms first
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bm_[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bm_[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-                }
-        } else if constexpr (MODE == MODE_BF16) {
-            s8 af[2], bf[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    af[t][q] = to_bf16_bits((float)As[lk * 8 + q][wm + t * 16 + li]);
-                    bf[t][q] = to_bf16_bits((float)Bs[lk * 8 + q][wn + t * 16 + li]);
-                }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
-        } else {
+        {
 #pragma unroll
             for (int k4 = 0; k4 < BK; k4 += 4) {
                 TS af[2], bf[2];
@@ -172,6 +128,211 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, TS alpha
                     *cp = (beta == TS(0)) ? val : val + beta * *cp;
                 }
             }
+}
+
+// ---------------------------------------------------------------- rank-K update
+// C (m x n) -= A (m x K) * B (K x n), all column-major, K <= 64: the trailing
+// update of a blocked LU.  Each workgroup owns a BM x BN tile of C:
+//   1. its C values are loaded straight into the MFMA accumulators, issued
+//      first so they arrive while the operands are staged.  The MFMA computes
+//      the transposed block (first operand from B, second from -A), so the 16
+//      lanes of a lane group hold 16 consecutive rows of one column of C and
+//      every load and store instruction covers 4 columns x 16 contiguous rows;
+//   2. -A (BM x K) and B (K x BN) are staged in LDS in one pass, every load
+//      of a thread in flight together (clamped in-bounds addresses, zeros
+//      selected past the edges);
+//   3. all K steps run back to back with no further barrier;
+//   4. the accumulators are stored.
+// FP32/BF16 modes: 128 x 128 tiles, each wavefront 64 x 64 (4 x 4 MFMA 16 x 16
+// blocks); FP64: 64 x 64 tiles, each wavefront 32 x 32.
+template <int MODE> struct RankTile { static constexpr int BM = 128, BN = 128; };
+template <> struct RankTile<MODE_F64> { static constexpr int BM = 64, BN = 64; };
+
+// output row (first-operand index) of a 16 x 16 MFMA block held by lane
+// group lk as value v; the output column is the lane index mod 16
+template <int MODE>
+__device__ __forceinline__ int acc_row(int lk, int v)
+{
+    return MODE == MODE_F64 ? lk + 4 * v : 4 * lk + v;
+}
+
+template <typename TS, int MODE>
+__global__ __launch_bounds__(256) void rank_update_kernel(int m, int n, int K, const TS *__restrict__ A, long lda,
+                                                          const TS *__restrict__ B, long ldb, TS *__restrict__ C,
+                                                          long ldc)
+{
+    constexpr int BM = RankTile<MODE>::BM, BN = RankTile<MODE>::BN, KM = 64;
+    constexpr int FM = BM / 32, FN = BN / 32;  // 16 x 16 blocks per wavefront and dimension
+    using AccT = typename Acc<MODE>::T;
+    // row strides of 16 banks (mod 32) keep the fragment reads conflict-free
+    constexpr int PAD = sizeof(TS) == 8 ? 8 : 16;
+    __shared__ TS As[KM][BM + PAD];  // As[k][i] = -A(bm + i, k)
+    __shared__ TS Bs[KM][BN + PAD];  // Bs[k][j] = B(k, bn + j)
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
+    const int wm = (w & 1) * (BM / 2), wn = (w >> 1) * (BN / 2);
+    const int li = lane & 15, lk = lane >> 4;
+
+    // 1. C into the accumulators: acc[a][b][v] of lane (li, lk) is
+    //    C(bm + wm + 16 a + li, bn + wn + 16 b + acc_row(lk, v))
+    AccT acc[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+        const int row = min(bm + wm + a * 16 + li, m - 1);
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = min(bn + wn + b * 16 + acc_row<MODE>(lk, v), n - 1);
+                acc[a][b][v] = C[row + (long)col * ldc];
+            }
+    }
+
+    // 2. operands into LDS
+    {
+        constexpr int RA = 256 / BM;  // columns of A per round
+        const int i = tid % BM, k0 = tid / BM;
+        const TS *ap = A + min(bm + i, m - 1);
+        const bool rok = bm + i < m;
+#pragma unroll
+        for (int c0 = 0; c0 < KM; c0 += 16 * RA) {
+            TS v[16];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int k = k0 + c0 + RA * s;
+                v[s] = ap[(long)min(k, K - 1) * lda];
+            }
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int k = k0 + c0 + RA * s;
+                As[k][i] = (rok && k < K) ? -v[s] : TS(0);
+            }
+        }
+    }
+    {
+        const int k = tid & 63, j0 = tid >> 6;
+        const TS *bp = B + min(k, K - 1);
+#pragma unroll
+        for (int c0 = 0; c0 < BN; c0 += 64) {
+            TS v[16];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int j = j0 + c0 + 4 * s;
+                v[s] = bp[(long)min(bn + j, n - 1) * ldb];
+            }
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int j = j0 + c0 + 4 * s;
+                Bs[k][j] = (k < K && bn + j < n) ? v[s] : TS(0);
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. all K steps
+    if constexpr (MODE == MODE_F64 || MODE == MODE_F32) {
+#pragma unroll 4
+        for (int k4 = 0; k4 < KM; k4 += 4) {
+            TS af[FM], bf[FN];
+#pragma unroll
+            for (int a = 0; a < FM; ++a) af[a] = As[k4 + lk][wm + a * 16 + li];
+#pragma unroll
+            for (int b = 0; b < FN; ++b) bf[b] = Bs[k4 + lk][wn + b * 16 + li];
+#pragma unroll
+            for (int a = 0; a < FM; ++a)
+#pragma unroll
+                for (int b = 0; b < FN; ++b) {
+                    if constexpr (MODE == MODE_F64)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[b], af[a], acc[a][b], 0, 0, 0);
+                }
+        }
+    } else {
+#pragma unroll 1
+        for (int k32 = 0; k32 < KM; k32 += 32) {
+            if constexpr (MODE == MODE_BF16) {
+                s8 af[FM], bf[FN];
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) af[a][q] = to_bf16_bits((float)As[k32 + lk * 8 + q][wm + a * 16 + li]);
+#pragma unroll
+                for (int b = 0; b < FN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) bf[b][q] = to_bf16_bits((float)Bs[k32 + lk * 8 + q][wn + b * 16 + li]);
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[b], af[a], acc[a][b], 0, 0, 0);
+            } else {  // MODE_BF16X3: three-way split, six cross products, smallest first
+                s8 ah[FM], am[FM], al[FM], bh[FN], bmd[FN], bl[FN];
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float x = (float)As[k32 + lk * 8 + q][wm + a * 16 + li];
+                        const short h0 = to_bf16_bits(x);
+                        const float r1 = x - bf16_bits_to_float(h0);
+                        const short h1 = to_bf16_bits(r1);
+                        ah[a][q] = h0;
+                        am[a][q] = h1;
+                        al[a][q] = to_bf16_bits(r1 - bf16_bits_to_float(h1));
+                    }
+#pragma unroll
+                for (int b = 0; b < FN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float y = (float)Bs[k32 + lk * 8 + q][wn + b * 16 + li];
+                        const short g0 = to_bf16_bits(y);
+                        const float s1 = y - bf16_bits_to_float(g0);
+                        const short g1 = to_bf16_bits(s1);
+                        bh[b][q] = g0;
+                        bmd[b][q] = g1;
+                        bl[b][q] = to_bf16_bits(s1 - bf16_bits_to_float(g1));
+                    }
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], am[a], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], ah[a], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], am[a], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], acc[a][b], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    // 4. store
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+        const int row = bm + wm + a * 16 + li;
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = bn + wn + b * 16 + acc_row<MODE>(lk, v);
+                if (row < m && col < n) C[row + (long)col * ldc] = acc[a][b][v];
+            }
+    }
+}
+
+template <typename TS, int MODE>
+hipError_t rank_update(hipStream_t st, int m, int n, int K, const TS *A, long lda, const TS *B, long ldb, TS *C,
+                       long ldc)
+{
+    if (m <= 0 || n <= 0 || K <= 0) return hipSuccess;
+    if (K > 64) return hipErrorInvalidValue;
+    constexpr int BM = RankTile<MODE>::BM, BN = RankTile<MODE>::BN;
+    dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+    hipLaunchKernelGGL((rank_update_kernel<TS, MODE>), grid, dim3(256), 0, st, m, n, K, A, lda, B, ldb, C, ldc);
+    return hipGetLastError();
 }
 
 // Launches C = alpha*A*B + beta*C on `st`.  Returns a hipError_t.

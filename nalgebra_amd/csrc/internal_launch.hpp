@@ -10,13 +10,12 @@ hipError_t nrocm_gemm_d(hipStream_t st, int m, int n, int k, double alpha, const
 hipError_t nrocm_gemm_s(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
                         long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
                         long csc);
-hipError_t nrocm_gemm_s_bf16(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
-                             long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
-                             long csc);
-
-hipError_t nrocm_gemm_s_bf16x3(hipStream_t st, int m, int n, int k, float alpha, const float *A, long rsa,
-                               long csa, const float *B, long rsb, long csb, float beta, float *C, long rsc,
-                               long csc);
+// C (m x n) -= A (m x k) * B (k x n), column-major, k <= 64 (LU trailing update).
+// mode: 1 = FP32 MFMA, 2 = operands rounded to BF16, 3 = three-way split BF16.
+hipError_t nrocm_rank_update_d(hipStream_t st, int m, int n, int k, const double *A, long lda, const double *B,
+                               long ldb, double *C, long ldc);
+hipError_t nrocm_rank_update_s(hipStream_t st, int m, int n, int k, const float *A, long lda, const float *B,
+                               long ldb, float *C, long ldc, int mode);
 
 // lu.hip: trailing-update flavours of the blocked LU.
 enum NrocmTrailing { TRAIL_EXACT = 0, TRAIL_F64 = 1, TRAIL_F32 = 2, TRAIL_BF16 = 3, TRAIL_BF16X3 = 4 };

@@ -104,17 +104,11 @@ hipError_t trailing(hipStream_t st, NrocmTrailing trail, T *A, int lda, int r0, 
     const T *Up = A + k0 + (size_t)c0 * lda;
     if (trail == TRAIL_EXACT) return simt_update<T, true, false>(st, C, lda, mm, nn, Lp, lda, Up, 1, lda, K, 0);
     if constexpr (sizeof(T) == 8) {
-        return nrocm_gemm_d(st, mm, nn, K, -1.0, (const double *)Lp, 1, lda, (const double *)Up, 1, lda, 1.0,
-                            (double *)C, 1, lda);
+        return nrocm_rank_update_d(st, mm, nn, K, (const double *)Lp, lda, (const double *)Up, lda, (double *)C, lda);
     } else {
-        if (trail == TRAIL_BF16X3)
-            return nrocm_gemm_s_bf16x3(st, mm, nn, K, -1.0f, (const float *)Lp, 1, lda, (const float *)Up, 1, lda,
-                                       1.0f, (float *)C, 1, lda);
-        if (trail == TRAIL_BF16)
-            return nrocm_gemm_s_bf16(st, mm, nn, K, -1.0f, (const float *)Lp, 1, lda, (const float *)Up, 1, lda,
-                                     1.0f, (float *)C, 1, lda);
-        return nrocm_gemm_s(st, mm, nn, K, -1.0f, (const float *)Lp, 1, lda, (const float *)Up, 1, lda, 1.0f,
-                            (float *)C, 1, lda);
+        const int mode = trail == TRAIL_BF16X3 ? 3 : trail == TRAIL_BF16 ? 2 : 1;
+        return nrocm_rank_update_s(st, mm, nn, K, (const float *)Lp, lda, (const float *)Up, lda, (float *)C, lda,
+                                   mode);
     }
 }
 
