@@ -349,11 +349,34 @@ __global__ __launch_bounds__(256) void potrs_kernel(const double *L, int lda, in
     }
 }
 
+// The panel step of the windowed LU for the block at k0: the window factor,
+// then A21 U11^{-1} for the rows below the window.  Touches only the block's
+// own columns, besides ipiv, perm, info and the row plan.
+template <typename T>
+hipError_t window_panel(hipStream_t st, T *A, int lda, int n, int k0, int *ipiv, int *info, int *plan, int *perm)
+{
+    const int nb = std::min(64, n - k0);
+    const int kw = std::min(n, k0 + WINDOW_ROWS);
+    hipLaunchKernelGGL((window_panel_kernel<T>), dim3(1), dim3(WINDOW_ROWS), 0, st, A, lda, k0, nb, kw, ipiv, info,
+                       plan, perm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || kw >= n) return e;
+    hipLaunchKernelGGL((trsm_right_upper_kernel<T>), dim3(nrocm_cdiv(n - kw, TRSM_RU_THREADS)),
+                       dim3(TRSM_RU_THREADS), 0, st, A, lda, k0, nb, kw, n);
+    return hipGetLastError();
+}
+
 // Blocked LU with the pivot search of each 64-column block restricted to a
 // window of WINDOW_ROWS rows (window_panel_kernel), then the window's row
 // plan on the other columns, A21 U11^{-1} for the rows below the window, U12
 // and the trailing update.  ipiv has the LAPACK meaning, as for getrf_impl;
 // perm (optional) receives the composed row permutation.
+//
+// Look-ahead of depth one: the panel step of block k+1 needs only the
+// trailing update of its own 64 columns, so that part of block k's update
+// runs first and the panel step then runs on an auxiliary stream beside the
+// update of the remaining columns.  The row plan of block k+1 is applied
+// after both have finished (it touches those columns too).
 template <typename T>
 int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int *ipiv, int *perm, int *info,
                       NrocmTrailing trail)
@@ -366,28 +389,26 @@ int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int 
     }
     int *plan = (int *)h->small + NROCM_SMALL_PLAN;
     constexpr int NB = 64;
+    LU_TRY(window_panel<T>(st, A, lda, n, 0, ipiv, info, plan, perm));
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = std::min(NB, n - k0);
         const int k1 = k0 + nb;
-        const int kw = std::min(n, k0 + WINDOW_ROWS);
-        hipLaunchKernelGGL((window_panel_kernel<T>), dim3(1), dim3(WINDOW_ROWS), 0, st, A, lda, k0, nb, kw, ipiv,
-                           info, plan, perm);
-        LU_TRY(hipGetLastError());
         const int ncol = k0 + (n - k1);
         if (ncol > 0) {
             hipLaunchKernelGGL((apply_plan_kernel<T>), dim3(nrocm_cdiv(ncol, 2)), dim3(256), 0, st, A, lda, plan, 0,
                                k0, k1, n);
             LU_TRY(hipGetLastError());
         }
-        if (kw < n) {
-            hipLaunchKernelGGL((trsm_right_upper_kernel<T>), dim3(nrocm_cdiv(n - kw, TRSM_RU_THREADS)),
-                               dim3(TRSM_RU_THREADS), 0, st, A, lda, k0, nb, kw, n);
-            LU_TRY(hipGetLastError());
-        }
-        if (k1 < n) {
-            LU_TRY(trsm<T>(st, A, lda, k0, nb, k1, n, false));
-            LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k1, n, k0, nb));
-        }
+        if (k1 >= n) break;
+        LU_TRY(trsm<T>(st, A, lda, k0, nb, k1, n, false));
+        const int k2 = std::min(n, k1 + NB);
+        LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k1, k2, k0, nb));
+        int s = nrocm_fork_on(h, st, 1);
+        if (s != NROCM_OK) return s;
+        LU_TRY(window_panel<T>(h->aux[0], A, lda, n, k1, ipiv, info, plan, perm));
+        LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k2, n, k0, nb));
+        s = nrocm_join_on(h, st, 1);
+        if (s != NROCM_OK) return s;
     }
     return NROCM_OK;
 }
@@ -427,6 +448,8 @@ template <typename T>
 int getrf_window_graph(nrocm_handle h, int n, T *A, int lda, int *ipiv, int *perm, int *info, NrocmTrailing trail)
 {
     if (n <= 64) return getrf_window_impl<T>(h, h->stream, n, A, lda, ipiv, perm, info, trail);
+    const int s = nrocm_aux_init(h);  // streams and events exist before the capture
+    if (s != NROCM_OK) return s;
     const long long key[8] = {3000 + (long long)sizeof(T) + 16 * (long long)trail, n, lda, (long long)A,
                               (long long)ipiv, (long long)perm, (long long)info, 0};
     return nrocm_graph_run(

@@ -574,6 +574,33 @@ __global__ __launch_bounds__(TRSM_RU_THREADS) void trsm_right_upper_kernel(T *A,
 // leaves one workgroup.
 constexpr int WINDOW_ROWS = 128;
 
+// Pivot key of window position pos (< 128) for a magnitude av: its top bits
+// with the low 7 replaced by 127 - pos.  The unsigned maximum of the keys is
+// the largest magnitude to within 2^-16 relative (2^-13 for FP64) and, among
+// magnitudes that close, the lowest position: a pivot within that factor of
+// the largest is as good for this solver, and the search becomes one integer
+// maximum per DPP step.
+template <typename T>
+__device__ __forceinline__ unsigned pivot_key(T av, int pos)
+{
+    const unsigned hi =
+        sizeof(T) == 8 ? (unsigned)((unsigned long long)__double_as_longlong((double)av) >> 32) : __float_as_uint((float)av);
+    return (hi & ~127u) | (unsigned)(127 - pos);
+}
+
+// Wavefront maximum of an unsigned value; lane 63 ends with it (same DPP
+// pattern as argmax_wave).
+__device__ __forceinline__ unsigned wave_max_u32(unsigned k)
+{
+    k = max(k, (unsigned)dpp32<0x111, 0xf>(0, (int)k));
+    k = max(k, (unsigned)dpp32<0x112, 0xf>(0, (int)k));
+    k = max(k, (unsigned)dpp32<0x114, 0xf>(0, (int)k));
+    k = max(k, (unsigned)dpp32<0x118, 0xf>(0, (int)k));
+    k = max(k, (unsigned)dpp32<0x142, 0xa>(0, (int)k));
+    k = max(k, (unsigned)dpp32<0x143, 0xc>(0, (int)k));
+    return k;
+}
+
 // One thread per window row, the row's nb values in registers.  Rows are not
 // moved during the factorisation: each thread carries the logical position of
 // its row, only the pivot row passes through LDS, and every row is written to
@@ -587,8 +614,7 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
                                                                    int *info, int *plan, int *perm)
 {
     __shared__ T prow[2][64];
-    __shared__ T wv[2][2];
-    __shared__ int wi[2][2];
+    __shared__ unsigned wk[2][2];
     __shared__ unsigned long long wmask[2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = kw - k0;
@@ -599,25 +625,21 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
     for (int q = 0; q < 64; ++q) a[q] = (live && q < nb) ? row[(size_t)(k0 + q) * lda] : T(0);
     const int myperm = (perm && live) ? perm[k0 + t] : 0;
     int pos = t;
+    int nbl = nb;
+    asm volatile("" : "+s"(nbl));  // separate from the load masks, which then need not stay alive
 #pragma unroll
     for (int c = 0; c < 64; ++c) {
-        if (c < nb) {  // uniform
+        if (c < nbl) {  // uniform
             const int par = c & 1;
-            T bv = (live && pos >= c) ? fabs(a[c]) : T(-1);
-            int bi = pos;
-            argmax_wave<T>(bv, bi);
-            if (lane == 63) {
-                wv[par][wave] = bv;
-                wi[par][wave] = bi;
-            }
+            unsigned key = (live && pos >= c) ? pivot_key(fabs(a[c]), pos) : 0u;
+            key = wave_max_u32(key);
+            if (lane == 63) wk[par][wave] = key;
             lds_barrier();
-            T pv = wv[par][0];
-            int P = wi[par][0];
-            if (beats_v(wv[par][1], wi[par][1], pv, P)) {
-                pv = wv[par][1];
-                P = wi[par][1];
-            }
-            const bool zero = !(pv > T(0));
+            const unsigned kmax = max(wk[par][0], wk[par][1]);
+            int P = 127 - (int)(kmax & 127u);
+            const unsigned mag = kmax & ~127u;
+            // an all-zero (or non-finite) column: nothing to exchange or eliminate
+            const bool zero = mag == 0u || mag >= (sizeof(T) == 8 ? 0x7ff00000u : 0x7f800000u);
             if (zero) {
                 P = c;  // nothing to exchange or eliminate
                 if (t == 0 && *info == 0) *info = k0 + c + 1;
@@ -642,10 +664,12 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
     }
     if (perm && live) perm[k0 + pos] = myperm;
     if (live) {
+        int nbs = nb;
+        asm volatile("" : "+s"(nbs));
         T *dst = A + (k0 + pos);
 #pragma unroll
         for (int q = 0; q < 64; ++q)
-            if (q < nb) dst[(size_t)(k0 + q) * lda] = a[q];
+            if (q < nbs) dst[(size_t)(k0 + q) * lda] = a[q];
     }
     // row plan: the window row now at position pos came from position t
     const bool moved = live && pos != t;

@@ -99,6 +99,12 @@ int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr);
 // with the other auxiliary streams and after everything queued before the fork.
 int nrocm_fork(nrocm_handle h);
 int nrocm_join(nrocm_handle h);
+// The same for the first `count` auxiliary streams and any origin stream st
+// (also a stream under graph capture: the auxiliary work joins the capture;
+// call nrocm_aux_init before the capture begins so that it creates nothing).
+int nrocm_aux_init(nrocm_handle h);
+int nrocm_fork_on(nrocm_handle h, hipStream_t st, int count);
+int nrocm_join_on(nrocm_handle h, hipStream_t st, int count);
 
 #define NROCM_CHECK_HIP(h, call)                                    \
     do {                                                            \

@@ -40,7 +40,7 @@ int nrocm_workspace(nrocm_handle h, size_t bytes, void **ptr)
     return NROCM_OK;
 }
 
-int nrocm_fork(nrocm_handle h)
+int nrocm_aux_init(nrocm_handle h)
 {
     if (!h->fork_ev) {
         NROCM_CHECK_HIP(h, hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
@@ -49,19 +49,30 @@ int nrocm_fork(nrocm_handle h)
             NROCM_CHECK_HIP(h, hipEventCreateWithFlags(&h->join_ev[i], hipEventDisableTiming));
         }
     }
-    NROCM_CHECK_HIP(h, hipEventRecord(h->fork_ev, h->stream));
-    for (int i = 0; i < nrocm_handle_s::kAux; ++i) NROCM_CHECK_HIP(h, hipStreamWaitEvent(h->aux[i], h->fork_ev, 0));
     return NROCM_OK;
 }
 
-int nrocm_join(nrocm_handle h)
+int nrocm_fork_on(nrocm_handle h, hipStream_t st, int count)
 {
-    for (int i = 0; i < nrocm_handle_s::kAux; ++i) {
+    const int s = nrocm_aux_init(h);
+    if (s != NROCM_OK) return s;
+    NROCM_CHECK_HIP(h, hipEventRecord(h->fork_ev, st));
+    for (int i = 0; i < count; ++i) NROCM_CHECK_HIP(h, hipStreamWaitEvent(h->aux[i], h->fork_ev, 0));
+    return NROCM_OK;
+}
+
+int nrocm_join_on(nrocm_handle h, hipStream_t st, int count)
+{
+    for (int i = 0; i < count; ++i) {
         NROCM_CHECK_HIP(h, hipEventRecord(h->join_ev[i], h->aux[i]));
-        NROCM_CHECK_HIP(h, hipStreamWaitEvent(h->stream, h->join_ev[i], 0));
+        NROCM_CHECK_HIP(h, hipStreamWaitEvent(st, h->join_ev[i], 0));
     }
     return NROCM_OK;
 }
+
+int nrocm_fork(nrocm_handle h) { return nrocm_fork_on(h, h->stream, nrocm_handle_s::kAux); }
+
+int nrocm_join(nrocm_handle h) { return nrocm_join_on(h, h->stream, nrocm_handle_s::kAux); }
 
 extern "C" {
 
