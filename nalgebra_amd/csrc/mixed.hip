@@ -173,6 +173,9 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
             dst[q] = (rr < rows && c < cols) ? src[(size_t)c * lda] : T(0);
         }
     };
+    // this block's own right-hand side: written only by this workgroup, so it
+    // is loaded now rather than on the chain's critical path
+    const T vown = (tid < rows) ? v[r0 + tid] : T(0);
     T acc = 0;
     T nxt[PER];
     if (k > 0) load_block(0, nxt);
@@ -203,8 +206,7 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
     }
     part[sl][rr] = acc;
     __syncthreads();
-    if (tid < BS) ys[tid] = (tid < rows) ? v[r0 + tid] + ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]))
-                                         : T(0);
+    if (tid < BS) ys[tid] = (tid < rows) ? vown + ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid])) : T(0);
     __syncthreads();
     T sacc = 0;
 #pragma unroll
