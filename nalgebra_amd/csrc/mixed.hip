@@ -135,13 +135,30 @@ __global__ __launch_bounds__(64) void inv_diag_kernel(const T *__restrict__ M, i
 // U y = v (non-unit upper) with the inverted diagonal blocks.  Work items are
 // 64-row blocks, taken in solve order through an atomic ticket, so every block
 // a workgroup waits for is owned by a workgroup that is already running.  The
-// off-diagonal block of step j+1 is loaded while step j waits.  Waits are
+// off-diagonal block of step j+1 is loaded while step j waits.
+//
+// A solved block is published in `tagged`: every 32-bit half of a value sits
+// in one 64-bit word next to this launch's epoch, stored as one atomic word.
+// A consumer lane polls its own words until they carry the epoch, so the data
+// and its readiness arrive together (no flag, no fence).  The array is zeroed
+// before the first launch of a solve and epochs are never reused on a handle,
+// so no word carries the current epoch before its value is written.  Waits are
 // bounded: on timeout the error word is set and the kernel still exits.
+template <typename T>
+__device__ __forceinline__ T untag(const unsigned long long (&w)[sizeof(T) / 4])
+{
+    if constexpr (sizeof(T) == 8)
+        return __longlong_as_double((long long)(((w[1] & 0xffffffffull) << 32) | (w[0] & 0xffffffffull)));
+    else
+        return __uint_as_float((unsigned)w[0]);
+}
+
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict__ M, int lda, int n,
                                                             const T *__restrict__ Inv, T *v, int *ticket,
-                                                            int *flags, int *err)
+                                                            unsigned long long *tagged, unsigned epoch, int *err)
 {
+    constexpr int WPV = sizeof(T) / 4;  // tagged words per value
     constexpr int PER = BS / 4;  // columns per thread and block
     __shared__ T xs[BS];
     __shared__ T part[4][BS];
@@ -187,18 +204,24 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
 #pragma unroll
         for (int q = 0; q < PER; ++q) cur[q] = nxt[q];
         if (j + 1 < k) load_block(j + 1, nxt);
-        if (tid == 0) {
-            long spins = 0;
-            while (__hip_atomic_load(&flags[J], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                if (++spins > SPIN_LIMIT) {
-                    atomicExch(err, 1);
-                    break;
+        if (tid < cols) {
+            unsigned long long w[WPV];
+#pragma unroll
+            for (int h = 0; h < WPV; ++h) {
+                unsigned long long *src = tagged + (size_t)(c0 + tid) * WPV + h;
+                long spins = 0;
+                unsigned long long word;
+                while (((word = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != epoch) {
+                    if (++spins > SPIN_LIMIT) {
+                        atomicExch(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_s_sleep(1);
+                w[h] = word;
             }
+            xs[tid] = untag<T>(w);
         }
-        __syncthreads();
-        if (tid < cols) xs[tid] = __hip_atomic_load(&v[c0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) acc = fma(-cur[q], xs[sl + 4 * q], acc);  // cur is 0 outside the block
@@ -215,12 +238,15 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
     __syncthreads();
     if (tid < rows) {
         const T y = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
-        __hip_atomic_store(&v[r0 + tid], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        __hip_atomic_store(&flags[R], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        v[r0 + tid] = y;
+        unsigned long long bits;
+        if constexpr (sizeof(T) == 8) bits = (unsigned long long)__double_as_longlong(y);
+        else bits = __float_as_uint(y);
+#pragma unroll
+        for (int h = 0; h < WPV; ++h)
+            __hip_atomic_store(tagged + (size_t)(r0 + tid) * WPV + h,
+                               ((unsigned long long)epoch << 32) | ((bits >> (32 * h)) & 0xffffffffull),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -291,6 +317,12 @@ __global__ void absmax_kernel(const double *__restrict__ v, int n, unsigned long
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
     const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void zero_words_kernel(unsigned long long *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
@@ -378,20 +410,21 @@ __global__ void widen_kernel(const T *__restrict__ d, int n, double *__restrict_
 
 template <typename T>
 int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const T *InvL, const T *InvU, T *v,
-              int *sync, int *err)
+              int *ticket, unsigned long long *tagged, int *err)
 {
     const int nblk = (n + BS - 1) / BS;
-    int *ticket = sync, *flags = sync + 1;
-    hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
-    IR_TRY(hipGetLastError());
-    hipLaunchKernelGGL((trsv_lookback_kernel<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvL, v, ticket,
-                       flags, err);
-    IR_TRY(hipGetLastError());
-    hipLaunchKernelGGL(zero_int_kernel, dim3(nrocm_cdiv(nblk + 1, 256)), dim3(256), 0, st, sync, nblk + 1);
-    IR_TRY(hipGetLastError());
-    hipLaunchKernelGGL((trsv_lookback_kernel<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvU, v, ticket,
-                       flags, err);
-    IR_TRY(hipGetLastError());
+    for (int pass = 0; pass < 2; ++pass) {
+        if (++h->trsv_epoch == 0) ++h->trsv_epoch;  // 0 is the zeroed array's tag
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, ticket, 1);
+        IR_TRY(hipGetLastError());
+        if (pass == 0)
+            hipLaunchKernelGGL((trsv_lookback_kernel<T, true>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvL, v,
+                               ticket, tagged, h->trsv_epoch, err);
+        else
+            hipLaunchKernelGGL((trsv_lookback_kernel<T, false>), dim3(nblk), dim3(256), 0, st, LU, lda, n, InvU, v,
+                               ticket, tagged, h->trsv_epoch, err);
+        IR_TRY(hipGetLastError());
+    }
     return NROCM_OK;
 }
 
@@ -405,8 +438,8 @@ int trsv_pair(nrocm_handle h, hipStream_t st, const T *LU, int lda, int n, const
 template <typename T>
 int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, const T *W, T *d,
            const int *perm, const T *InvL, const T *InvU, double *r,
-           double *part, int *sync, int *err, unsigned long long *red, const int *info, double *Vk, double *Zk,
-           double *hdev, double tol, int maxit, int *iters, double *berr_out)
+           double *part, int *sync, unsigned long long *tagged, int *err, unsigned long long *red, const int *info,
+           double *Vk, double *Zk, double *hdev, double tol, int maxit, int *iters, double *berr_out)
 {
     hipStream_t st = h->stream;
     constexpr int M = GM_RESTART;
@@ -458,7 +491,7 @@ int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, dou
             // z_j = M^{-1} v_j
             hipLaunchKernelGGL((gather_kernel<T>), dim3(nb), dim3(256), 0, st, vj, perm, d, n);
             IR_TRY(hipGetLastError());
-            s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
+            s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, tagged, err);
             if (s != NROCM_OK) return s;
             hipLaunchKernelGGL((widen_kernel<T>), dim3(nb), dim3(256), 0, st, d, n, zj);
             ++solves;
@@ -528,6 +561,7 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     const size_t oW = take(szW), oIpiv = take(n * sizeof(int)), oPerm = take(n * sizeof(int)),
                  oR = take(szVec), oD = take((size_t)n * sizeof(T)), oPart = take(GEMV_CS * szVec),
                  oSync = take((nblk + 2) * sizeof(int)), oRed = take(8 * sizeof(unsigned long long)),
+                 oTag = take((size_t)n * (sizeof(T) / 4) * sizeof(unsigned long long)),
                  oInfo = take(2 * sizeof(int)), oInv = take(2 * (size_t)nblk * BS * BS * sizeof(T)),
                  oKry = take(gmres ? (size_t)(2 * GM_RESTART + 1) * szVec + 4 * (GM_RESTART + 2) * sizeof(double) : 0);
     void *ws = nullptr;
@@ -539,6 +573,10 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     double *r = (double *)(base + oR), *part = (double *)(base + oPart);
     T *d = (T *)(base + oD);
     int *sync = (int *)(base + oSync);
+    unsigned long long *tagged = (unsigned long long *)(base + oTag);
+    const size_t ntag = (size_t)n * (sizeof(T) / 4);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(nrocm_cdiv(ntag, 256)), dim3(256), 0, st, tagged, ntag);
+    IR_TRY(hipGetLastError());
     unsigned long long *red = (unsigned long long *)(base + oRed);  // [0]=|A|, [1]=|b|, [2]=|r|, [3]=|x|
     int *info = (int *)(base + oInfo), *err = info + 1;
     T *InvL = (T *)(base + oInv), *InvU = InvL + (size_t)nblk * BS * BS;
@@ -584,12 +622,12 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     int it = 0;
     unsigned long long hred[4];
     int hinfo[2];
-    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, InvL, InvU, r, part, sync, err, red, info, Vk, Zk, hdev,
-                             tol, maxit, iters, berr_out);
+    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, InvL, InvU, r, part, sync, tagged, err, red, info, Vk, Zk,
+                             hdev, tol, maxit, iters, berr_out);
     for (it = 1; it <= maxit; ++it) {
         hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
         IR_TRY(hipGetLastError());
-        s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, err);
+        s = trsv_pair<T>(h, st, W, n, n, InvL, InvU, d, sync, tagged, err);
         if (s != NROCM_OK) return s;
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, red + 2, 2);
         hipLaunchKernelGGL((update_x_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, x, d, n, red + 3);

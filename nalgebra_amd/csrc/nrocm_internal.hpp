@@ -54,6 +54,9 @@ struct nrocm_handle_s {
     };
     std::vector<Graph> graphs;
     hipStream_t capture_stream = nullptr;
+    // Epoch of the last look-back triangular solve launched on this handle
+    // (tags its published values; never reused while the handle lives).
+    unsigned trsv_epoch = 0;
 };
 
 // Runs `body(stream)` through the handle's graph cache: on a miss the launches
