@@ -301,9 +301,10 @@ def bench_lu(args):
     }
     if not args.no_cpu:
         from oracle import oracle as orc
-        # the full n when it fits the CPU budget (about 25 s at n = 8192), else the
-        # leading quarter-size block scaled by (n / ncpu)^3
-        ncpu = n if (n <= 8192 and args.cpu_seconds >= 20) else min(n, 2048)
+        # a bounded sample: the largest leading block whose oracle LU fits the CPU
+        # budget (LU::new takes about 65 s at n = 8192 and 8 s at 4096 on one core),
+        # scaled by (n / ncpu)^3
+        ncpu = n if args.cpu_seconds >= 70 else min(n, 4096 if args.cpu_seconds >= 10 else 2048)
         Ah = np.ascontiguousarray(A.view(n, n).T[:ncpu, :ncpu].cpu().numpy())
         t0 = time.perf_counter()
         orc.getrf(Ah)
