@@ -515,6 +515,13 @@ __global__ __launch_bounds__(TRSM_RU_THREADS) void trsm_right_upper_kernel(T *A,
 {
     __shared__ __align__(16) T Ut[64][64];  // Ut[q][p] = U[p][q]
     const int t = threadIdx.x;
+    // the row to solve is loaded first, so that its loads overlap the staging
+    const int r = r0 + blockIdx.x * TRSM_RU_THREADS + t;
+    const bool live = r < m;
+    T *row = A + (live ? r : r0);
+    T x[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) x[q] = row[(size_t)(k0 + (q < nb ? q : 0)) * lda];
     // thread t stages row t of U; 16 loads in flight (clamped in-bounds
     // addresses, zeros selected afterwards) instead of one load per wait
     const T *urow = A + (k0 + (t < nb ? t : 0));
@@ -527,12 +534,8 @@ __global__ __launch_bounds__(TRSM_RU_THREADS) void trsm_right_upper_kernel(T *A,
         for (int j = 0; j < 16; ++j) Ut[q0 + j][t] = (t <= q0 + j && q0 + j < nb) ? v[j] : T(0);
     }
     __syncthreads();
-    const int r = r0 + blockIdx.x * TRSM_RU_THREADS + t;
-    const bool live = r < m;
-    T *row = A + (live ? r : r0);
-    T x[64];
-#pragma unroll
-    for (int q = 0; q < 64; ++q) x[q] = (live && q < nb) ? row[(size_t)(k0 + q) * lda] : T(0);
+    // x[q] past nb holds column 0 again and is never read: the solve below
+    // touches x[p] only for p < q < nb, and rows past m are not stored
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
         if (q < nb) {  // uniform
