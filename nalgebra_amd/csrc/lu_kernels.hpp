@@ -514,6 +514,7 @@ __global__ __launch_bounds__(TRSM_RU_THREADS) void trsm_right_upper_kernel(T *A,
                                                                            int m)
 {
     __shared__ __align__(16) T Ut[64][64];  // Ut[q][p] = U[p][q]
+    __builtin_amdgcn_s_setprio(3);  // on the look-ahead critical path, beside the trailing update
     const int t = threadIdx.x;
     // the row to solve is loaded first, so that its loads overlap the staging
     const int r = r0 + blockIdx.x * TRSM_RU_THREADS + t;
@@ -619,6 +620,9 @@ __global__ __launch_bounds__(WINDOW_ROWS) void window_panel_kernel(T *A, int lda
     __shared__ T prow[2][64];
     __shared__ unsigned wk[2][2];
     __shared__ unsigned long long wmask[2];
+    // this kernel is the critical path of the look-ahead pipeline and shares
+    // its CU with trailing-update wavefronts: issue first among them
+    __builtin_amdgcn_s_setprio(3);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = kw - k0;
     const bool live = t < wr;
