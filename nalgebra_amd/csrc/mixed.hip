@@ -204,23 +204,30 @@ __global__ __launch_bounds__(256) void trsv_lookback_kernel(const T *__restrict_
 #pragma unroll
         for (int q = 0; q < PER; ++q) cur[q] = nxt[q];
         if (j + 1 < k) load_block(j + 1, nxt);
-        if (tid < cols) {
-            unsigned long long w[WPV];
+        if (tid < BS) {
+            // every slot is written: entries past a short last block are zero
+            // (the products below read all BS slots, and 0 * stale LDS could be NaN)
+            T xv = T(0);
+            if (tid < cols) {
+                unsigned long long w[WPV];
 #pragma unroll
-            for (int h = 0; h < WPV; ++h) {
-                unsigned long long *src = tagged + (size_t)(c0 + tid) * WPV + h;
-                long spins = 0;
-                unsigned long long word;
-                while (((word = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != epoch) {
-                    if (++spins > SPIN_LIMIT) {
-                        atomicExch(err, 1);
-                        break;
+                for (int h = 0; h < WPV; ++h) {
+                    unsigned long long *src = tagged + (size_t)(c0 + tid) * WPV + h;
+                    long spins = 0;
+                    unsigned long long word;
+                    while (((word = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) !=
+                           epoch) {
+                        if (++spins > SPIN_LIMIT) {
+                            atomicExch(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    w[h] = word;
                 }
-                w[h] = word;
+                xv = untag<T>(w);
             }
-            xs[tid] = untag<T>(w);
+            xs[tid] = xv;
         }
         __syncthreads();
 #pragma unroll

@@ -50,7 +50,7 @@ def test_mixed_ir_converges(nl, orc, n, prec):
         assert iters <= 6
 
 
-@pytest.mark.parametrize("n", [300, 1024])
+@pytest.mark.parametrize("n", [300, 777, 1024])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_mixed_ir_window_converges(nl, orc, n, prec):
     """Pivots searched in the next 128 rows of each 64-column block only: same
@@ -71,6 +71,23 @@ def test_mixed_ir_window_converges(nl, orc, n, prec):
     fe = np.max(np.abs(xv - xr[:, 0])) / np.max(np.abs(xr))
     assert fe <= 1e4 * 1e-14, fe
     assert iters <= (2 if prec == "fp64" else 8)
+
+
+@pytest.mark.parametrize("n", [641, 656, 777])
+@pytest.mark.parametrize("window", [False, True])
+def test_mixed_ir_short_last_block(nl, n, window):
+    """A last 64-row block of 1, 16 or 9 rows: the backward triangular solve
+    consumes that short block in every other block (regression: stale LDS
+    slots past it were multiplied by zero, which gives NaN for a stale NaN)."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + n * np.eye(n)
+    b = rng.standard_normal(n)
+    p = nl.PREC_FP32 | (nl.PREC_WINDOW if window else 0)
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
+                                         tol=1e-14, maxit=10)
+    xv = x.numpy()[:, 0]
+    assert np.all(np.isfinite(xv))
+    assert ok and be <= 1e-14 and berr(A, xv, b) <= 2e-14
 
 
 def test_mixed_ir_window_singular(nl):
