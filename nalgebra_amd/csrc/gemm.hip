@@ -21,18 +21,18 @@ hipError_t nrocm_gemm_s(hipStream_t st, int m, int n, int k, float alpha, const 
 }
 
 hipError_t nrocm_rank_update_d(hipStream_t st, int m, int n, int k, const double *A, long lda, const double *B,
-                               long ldb, double *C, long ldc)
+                               long ldb, double *C, long ldc, int max_groups)
 {
-    return rank_update<double, MODE_F64>(st, m, n, k, A, lda, B, ldb, C, ldc);
+    return rank_update<double, MODE_F64>(st, m, n, k, A, lda, B, ldb, C, ldc, max_groups);
 }
 
 hipError_t nrocm_rank_update_s(hipStream_t st, int m, int n, int k, const float *A, long lda, const float *B,
-                               long ldb, float *C, long ldc, int mode)
+                               long ldb, float *C, long ldc, int mode, int max_groups)
 {
     switch (mode) {
-    case MODE_BF16: return rank_update<float, MODE_BF16>(st, m, n, k, A, lda, B, ldb, C, ldc);
-    case MODE_BF16X3: return rank_update<float, MODE_BF16X3>(st, m, n, k, A, lda, B, ldb, C, ldc);
-    default: return rank_update<float, MODE_F32>(st, m, n, k, A, lda, B, ldb, C, ldc);
+    case MODE_BF16: return rank_update<float, MODE_BF16>(st, m, n, k, A, lda, B, ldb, C, ldc, max_groups);
+    case MODE_BF16X3: return rank_update<float, MODE_BF16X3>(st, m, n, k, A, lda, B, ldb, C, ldc, max_groups);
+    default: return rank_update<float, MODE_F32>(st, m, n, k, A, lda, B, ldb, C, ldc, max_groups);
     }
 }
 

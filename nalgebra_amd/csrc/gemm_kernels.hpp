@@ -156,11 +156,14 @@ __device__ __forceinline__ int acc_row(int lk, int v)
     return MODE == MODE_F64 ? lk + 4 * v : 4 * lk + v;
 }
 
-template <typename TS, int MODE>
+template <typename TS, int MODE, bool LOOP>
 __global__ __launch_bounds__(256) void rank_update_kernel(int m, int n, int K, const TS *__restrict__ A, long lda,
                                                           const TS *__restrict__ B, long ldb, TS *__restrict__ C,
                                                           long ldc)
 {
+    // LOOP: tiles are taken in a grid-stride loop, so a launch may use fewer
+    // workgroups than tiles (and leave CUs to concurrent work); otherwise one
+    // tile per workgroup
     constexpr int BM = RankTile<MODE>::BM, BN = RankTile<MODE>::BN, KM = 64;
     constexpr int FM = BM / 32, FN = BN / 32;  // 16 x 16 blocks per wavefront and dimension
     using AccT = typename Acc<MODE>::T;
@@ -171,167 +174,178 @@ __global__ __launch_bounds__(256) void rank_update_kernel(int m, int n, int K, c
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
-    const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
-    const int wm = (w & 1) * (BM / 2), wn = (w >> 1) * (BN / 2);
-    const int li = lane & 15, lk = lane >> 4;
+    const int tm = (m + BM - 1) / BM;
+    const int ntiles = tm * ((n + BN - 1) / BN);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int bm = (tile % tm) * BM, bn = (tile / tm) * BN;
+        const int wm = (w & 1) * (BM / 2), wn = (w >> 1) * (BN / 2);
+        const int li = lane & 15, lk = lane >> 4;
 
-    // 1. C into the accumulators: acc[a][b][v] of lane (li, lk) is
-    //    C(bm + wm + 16 a + li, bn + wn + 16 b + acc_row(lk, v))
-    AccT acc[FM][FN];
+        // 1. C into the accumulators: acc[a][b][v] of lane (li, lk) is
+        //    C(bm + wm + 16 a + li, bn + wn + 16 b + acc_row(lk, v))
+        AccT acc[FM][FN];
 #pragma unroll
-    for (int a = 0; a < FM; ++a) {
-        const int row = min(bm + wm + a * 16 + li, m - 1);
+        for (int a = 0; a < FM; ++a) {
+            const int row = min(bm + wm + a * 16 + li, m - 1);
 #pragma unroll
-        for (int b = 0; b < FN; ++b)
+            for (int b = 0; b < FN; ++b)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = min(bn + wn + b * 16 + acc_row<MODE>(lk, v), n - 1);
-                acc[a][b][v] = C[row + (long)col * ldc];
-            }
-    }
-
-    // 2. operands into LDS
-    {
-        constexpr int RA = 256 / BM;  // columns of A per round
-        const int i = tid % BM, k0 = tid / BM;
-        const TS *ap = A + min(bm + i, m - 1);
-        const bool rok = bm + i < m;
-#pragma unroll
-        for (int c0 = 0; c0 < KM; c0 += 16 * RA) {
-            TS v[16];
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int k = k0 + c0 + RA * s;
-                v[s] = ap[(long)min(k, K - 1) * lda];
-            }
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int k = k0 + c0 + RA * s;
-                As[k][i] = (rok && k < K) ? -v[s] : TS(0);
-            }
-        }
-    }
-    {
-        const int k = tid & 63, j0 = tid >> 6;
-        const TS *bp = B + min(k, K - 1);
-#pragma unroll
-        for (int c0 = 0; c0 < BN; c0 += 64) {
-            TS v[16];
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int j = j0 + c0 + 4 * s;
-                v[s] = bp[(long)min(bn + j, n - 1) * ldb];
-            }
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int j = j0 + c0 + 4 * s;
-                Bs[k][j] = (k < K && bn + j < n) ? v[s] : TS(0);
-            }
-        }
-    }
-    __syncthreads();
-
-    // 3. all K steps
-    if constexpr (MODE == MODE_F64 || MODE == MODE_F32) {
-#pragma unroll 4
-        for (int k4 = 0; k4 < KM; k4 += 4) {
-            TS af[FM], bf[FN];
-#pragma unroll
-            for (int a = 0; a < FM; ++a) af[a] = As[k4 + lk][wm + a * 16 + li];
-#pragma unroll
-            for (int b = 0; b < FN; ++b) bf[b] = Bs[k4 + lk][wn + b * 16 + li];
-#pragma unroll
-            for (int a = 0; a < FM; ++a)
-#pragma unroll
-                for (int b = 0; b < FN; ++b) {
-                    if constexpr (MODE == MODE_F64)
-                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
-                    else
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[b], af[a], acc[a][b], 0, 0, 0);
+                for (int v = 0; v < 4; ++v) {
+                    const int col = min(bn + wn + b * 16 + acc_row<MODE>(lk, v), n - 1);
+                    acc[a][b][v] = C[row + (long)col * ldc];
                 }
         }
-    } else {
-#pragma unroll 1
-        for (int k32 = 0; k32 < KM; k32 += 32) {
-            if constexpr (MODE == MODE_BF16) {
-                s8 af[FM], bf[FN];
+
+        // 2. operands into LDS
+        {
+            constexpr int RA = 256 / BM;  // columns of A per round
+            const int i = tid % BM, k0 = tid / BM;
+            const TS *ap = A + min(bm + i, m - 1);
+            const bool rok = bm + i < m;
 #pragma unroll
-                for (int a = 0; a < FM; ++a)
+            for (int c0 = 0; c0 < KM; c0 += 16 * RA) {
+                TS v[16];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) af[a][q] = to_bf16_bits((float)As[k32 + lk * 8 + q][wm + a * 16 + li]);
+                for (int s = 0; s < 16; ++s) {
+                    const int k = k0 + c0 + RA * s;
+                    v[s] = ap[(long)min(k, K - 1) * lda];
+                }
 #pragma unroll
-                for (int b = 0; b < FN; ++b)
+                for (int s = 0; s < 16; ++s) {
+                    const int k = k0 + c0 + RA * s;
+                    As[k][i] = (rok && k < K) ? -v[s] : TS(0);
+                }
+            }
+        }
+        {
+            const int k = tid & 63, j0 = tid >> 6;
+            const TS *bp = B + min(k, K - 1);
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) bf[b][q] = to_bf16_bits((float)Bs[k32 + lk * 8 + q][wn + b * 16 + li]);
+            for (int c0 = 0; c0 < BN; c0 += 64) {
+                TS v[16];
 #pragma unroll
-                for (int a = 0; a < FM; ++a)
+                for (int s = 0; s < 16; ++s) {
+                    const int j = j0 + c0 + 4 * s;
+                    v[s] = bp[(long)min(bn + j, n - 1) * ldb];
+                }
 #pragma unroll
-                    for (int b = 0; b < FN; ++b)
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[b], af[a], acc[a][b], 0, 0, 0);
-            } else {  // MODE_BF16X3: three-way split, six cross products, smallest first
-                s8 ah[FM], am[FM], al[FM], bh[FN], bmd[FN], bl[FN];
+                for (int s = 0; s < 16; ++s) {
+                    const int j = j0 + c0 + 4 * s;
+                    Bs[k][j] = (k < K && bn + j < n) ? v[s] : TS(0);
+                }
+            }
+        }
+        __syncthreads();
+
+        // 3. all K steps
+        if constexpr (MODE == MODE_F64 || MODE == MODE_F32) {
+#pragma unroll 4
+            for (int k4 = 0; k4 < KM; k4 += 4) {
+                TS af[FM], bf[FN];
 #pragma unroll
-                for (int a = 0; a < FM; ++a)
+                for (int a = 0; a < FM; ++a) af[a] = As[k4 + lk][wm + a * 16 + li];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const float x = (float)As[k32 + lk * 8 + q][wm + a * 16 + li];
-                        const short h0 = to_bf16_bits(x);
-                        const float r1 = x - bf16_bits_to_float(h0);
-                        const short h1 = to_bf16_bits(r1);
-                        ah[a][q] = h0;
-                        am[a][q] = h1;
-                        al[a][q] = to_bf16_bits(r1 - bf16_bits_to_float(h1));
-                    }
-#pragma unroll
-                for (int b = 0; b < FN; ++b)
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const float y = (float)Bs[k32 + lk * 8 + q][wn + b * 16 + li];
-                        const short g0 = to_bf16_bits(y);
-                        const float s1 = y - bf16_bits_to_float(g0);
-                        const short g1 = to_bf16_bits(s1);
-                        bh[b][q] = g0;
-                        bmd[b][q] = g1;
-                        bl[b][q] = to_bf16_bits(s1 - bf16_bits_to_float(g1));
-                    }
+                for (int b = 0; b < FN; ++b) bf[b] = Bs[k4 + lk][wn + b * 16 + li];
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
                     for (int b = 0; b < FN; ++b) {
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], am[a], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], ah[a], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], am[a], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], acc[a][b], 0, 0, 0);
+                        if constexpr (MODE == MODE_F64)
+                            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+                        else
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[b], af[a], acc[a][b], 0, 0, 0);
                     }
             }
-        }
-    }
-
-    // 4. store
+        } else {
+#pragma unroll 1
+            for (int k32 = 0; k32 < KM; k32 += 32) {
+                if constexpr (MODE == MODE_BF16) {
+                    s8 af[FM], bf[FN];
 #pragma unroll
-    for (int a = 0; a < FM; ++a) {
-        const int row = bm + wm + a * 16 + li;
+                    for (int a = 0; a < FM; ++a)
 #pragma unroll
-        for (int b = 0; b < FN; ++b)
+                        for (int q = 0; q < 8; ++q) af[a][q] = to_bf16_bits((float)As[k32 + lk * 8 + q][wm + a * 16 + li]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = bn + wn + b * 16 + acc_row<MODE>(lk, v);
-                if (row < m && col < n) C[row + (long)col * ldc] = acc[a][b][v];
+                    for (int b = 0; b < FN; ++b)
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) bf[b][q] = to_bf16_bits((float)Bs[k32 + lk * 8 + q][wn + b * 16 + li]);
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[b], af[a], acc[a][b], 0, 0, 0);
+                } else {  // MODE_BF16X3: three-way split, six cross products, smallest first
+                    s8 ah[FM], am[FM], al[FM], bh[FN], bmd[FN], bl[FN];
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const float x = (float)As[k32 + lk * 8 + q][wm + a * 16 + li];
+                            const short h0 = to_bf16_bits(x);
+                            const float r1 = x - bf16_bits_to_float(h0);
+                            const short h1 = to_bf16_bits(r1);
+                            ah[a][q] = h0;
+                            am[a][q] = h1;
+                            al[a][q] = to_bf16_bits(r1 - bf16_bits_to_float(h1));
+                        }
+#pragma unroll
+                    for (int b = 0; b < FN; ++b)
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const float y = (float)Bs[k32 + lk * 8 + q][wn + b * 16 + li];
+                            const short g0 = to_bf16_bits(y);
+                            const float s1 = y - bf16_bits_to_float(g0);
+                            const short g1 = to_bf16_bits(s1);
+                            bh[b][q] = g0;
+                            bmd[b][q] = g1;
+                            bl[b][q] = to_bf16_bits(s1 - bf16_bits_to_float(g1));
+                        }
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) {
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], am[a], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[b], ah[a], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], al[a], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bmd[b], ah[a], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], am[a], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[b], ah[a], acc[a][b], 0, 0, 0);
+                        }
+                }
             }
+        }
+
+        // 4. store
+#pragma unroll
+        for (int a = 0; a < FM; ++a) {
+            const int row = bm + wm + a * 16 + li;
+#pragma unroll
+            for (int b = 0; b < FN; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int col = bn + wn + b * 16 + acc_row<MODE>(lk, v);
+                    if (row < m && col < n) C[row + (long)col * ldc] = acc[a][b][v];
+                }
+        }
+        if constexpr (!LOOP) break;
+        __syncthreads();  // the next tile overwrites As and Bs
     }
 }
-
+// max_groups > 0 caps the workgroups (each then loops over tiles).
 template <typename TS, int MODE>
 hipError_t rank_update(hipStream_t st, int m, int n, int K, const TS *A, long lda, const TS *B, long ldb, TS *C,
-                       long ldc)
+                       long ldc, int max_groups)
 {
     if (m <= 0 || n <= 0 || K <= 0) return hipSuccess;
     if (K > 64) return hipErrorInvalidValue;
     constexpr int BM = RankTile<MODE>::BM, BN = RankTile<MODE>::BN;
-    dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
-    hipLaunchKernelGGL((rank_update_kernel<TS, MODE>), grid, dim3(256), 0, st, m, n, K, A, lda, B, ldb, C, ldc);
+    const long tiles = (long)((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    if (max_groups > 0 && tiles > max_groups)
+        hipLaunchKernelGGL((rank_update_kernel<TS, MODE, true>), dim3((unsigned)max_groups), dim3(256), 0, st, m, n,
+                           K, A, lda, B, ldb, C, ldc);
+    else
+        hipLaunchKernelGGL((rank_update_kernel<TS, MODE, false>), dim3((unsigned)tiles), dim3(256), 0, st, m, n, K,
+                           A, lda, B, ldb, C, ldc);
     return hipGetLastError();
 }
 

@@ -12,10 +12,11 @@ hipError_t nrocm_gemm_s(hipStream_t st, int m, int n, int k, float alpha, const 
                         long csc);
 // C (m x n) -= A (m x k) * B (k x n), column-major, k <= 64 (LU trailing update).
 // mode: 1 = FP32 MFMA, 2 = operands rounded to BF16, 3 = three-way split BF16.
+// max_groups > 0 caps the workgroups of the launch (0: one per 128x128 tile).
 hipError_t nrocm_rank_update_d(hipStream_t st, int m, int n, int k, const double *A, long lda, const double *B,
-                               long ldb, double *C, long ldc);
+                               long ldb, double *C, long ldc, int max_groups);
 hipError_t nrocm_rank_update_s(hipStream_t st, int m, int n, int k, const float *A, long lda, const float *B,
-                               long ldb, float *C, long ldc, int mode);
+                               long ldb, float *C, long ldc, int mode, int max_groups);
 
 // lu.hip: trailing-update flavours of the blocked LU.
 enum NrocmTrailing { TRAIL_EXACT = 0, TRAIL_F64 = 1, TRAIL_F32 = 2, TRAIL_BF16 = 3, TRAIL_BF16X3 = 4 };

@@ -95,7 +95,7 @@ hipError_t simt_update(hipStream_t st, T *C, int ldc, int m, int nc, const T *Lp
 // A22 (rows r0.., cols c0..n) -= L21 (rows r0.., cols k0..k0+K) * U12 (rows k0.., cols c0..)
 template <typename T>
 hipError_t trailing(hipStream_t st, NrocmTrailing trail, T *A, int lda, int r0, int m, int c0, int n, int k0,
-                    int K)
+                    int K, int max_groups = 0)
 {
     const int mm = m - r0, nn = n - c0;
     if (mm <= 0 || nn <= 0 || K <= 0) return hipSuccess;
@@ -104,11 +104,12 @@ hipError_t trailing(hipStream_t st, NrocmTrailing trail, T *A, int lda, int r0, 
     const T *Up = A + k0 + (size_t)c0 * lda;
     if (trail == TRAIL_EXACT) return simt_update<T, true, false>(st, C, lda, mm, nn, Lp, lda, Up, 1, lda, K, 0);
     if constexpr (sizeof(T) == 8) {
-        return nrocm_rank_update_d(st, mm, nn, K, (const double *)Lp, lda, (const double *)Up, lda, (double *)C, lda);
+        return nrocm_rank_update_d(st, mm, nn, K, (const double *)Lp, lda, (const double *)Up, lda, (double *)C, lda,
+                                   max_groups);
     } else {
         const int mode = trail == TRAIL_BF16X3 ? 3 : trail == TRAIL_BF16 ? 2 : 1;
         return nrocm_rank_update_s(st, mm, nn, K, (const float *)Lp, lda, (const float *)Up, lda, (float *)C, lda,
-                                   mode);
+                                   mode, max_groups);
     }
 }
 
@@ -406,7 +407,11 @@ int getrf_window_impl(nrocm_handle h, hipStream_t st, int n, T *A, int lda, int 
         int s = nrocm_fork_on(h, st, 1);
         if (s != NROCM_OK) return s;
         LU_TRY(window_panel<T>(h->aux[0], A, lda, n, k1, ipiv, info, plan, perm));
-        LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k2, n, k0, nb));
+        // FP32: two workgroups per CU on all but a few CUs, which stay free for
+        // the panel step running beside this update (measured: FP32 gains, the
+        // FP64 update with its smaller tiles does not)
+        LU_TRY(trailing<T>(st, trail, A, lda, k1, n, k2, n, k0, nb,
+                           sizeof(T) == 4 ? 2 * std::max(1, h->num_cus - 8) : 0));
         s = nrocm_join_on(h, st, 1);
         if (s != NROCM_OK) return s;
     }
