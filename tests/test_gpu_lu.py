@@ -51,6 +51,50 @@ def test_lu_exact_bitwise(nl, orc, shape):
         assert np.array_equal(_bits(x.numpy()), _bits(xr)), "solution bits differ"
 
 
+def _lu_case(kind, shape):
+    m, n = shape
+    rng = np.random.default_rng(7 * m + n)
+    if kind == "pm1":          # every first-column entry ties in magnitude
+        return rng.choice([-1.0, 1.0], size=shape)
+    if kind == "int2":         # entries in {-2..2}: ties in most pivot searches
+        return rng.integers(-2, 3, size=shape).astype(np.float64)
+    if kind == "zerocols":     # zero columns stay zero under elimination: zero pivots, info != 0
+        A = gen.uniform(300 + m, shape)
+        A[:, [5, n // 2, n - 3]] = 0.0
+        return A
+    return gen.uniform(400 + m + n, shape)
+
+
+@pytest.mark.parametrize("kind,shape", [("pm1", (64, 64)), ("pm1", (300, 300)), ("pm1", (700, 700)),
+                                        ("int2", (129, 129)), ("int2", (600, 600)), ("int2", (400, 250)),
+                                        ("zerocols", (90, 90)), ("zerocols", (260, 260)),
+                                        ("uniform", (500, 1)), ("uniform", (1, 500)),
+                                        ("uniform", (2000, 70)), ("uniform", (70, 2000))])
+def test_lu_exact_ties_and_shapes(nl, orc, kind, shape):
+    """Pivot ties keep the first row of greatest magnitude (icamax,
+    `src/base/min_max.rs:221-241`) and a zero pivot skips its step, with no
+    swap and no elimination (`lu.rs:107-110`): pivots, factors and info
+    identical to the oracle, bit for bit."""
+    m, n = shape
+    print(f"lu exact {kind} {shape}", flush=True)
+    A = _lu_case(kind, shape)
+    lu = nl.DMatrix.from_numpy(A).lu(nl.EXACT)
+    LUr, ipr, infor = orc.getrf(A)
+    k = min(m, n)
+    assert np.array_equal(lu.ipiv[:k].cpu().numpy(), ipr), "pivot sequence differs"
+    assert lu.info == infor
+    assert np.array_equal(_bits(lu.lu.numpy()), _bits(LUr)), "LU bits differ"
+    if kind == "zerocols":
+        assert infor != 0
+    if m == n and infor == 0:
+        b = gen.uniform(500 + n, (n, 3))
+        x = nl.DMatrix.from_numpy(b)
+        assert lu.solve_mut(x)
+        xr, ok = orc.getrs(LUr, ipr, b)
+        assert ok
+        assert np.array_equal(_bits(x.numpy()), _bits(xr)), "solution bits differ"
+
+
 def test_config1_gemm_lu_solve(nl, orc):
     """f64 512x512, entries U[-1,1]: C = A*B, LU of A, solve A x = b."""
     n = 512
