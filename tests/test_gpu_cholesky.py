@@ -101,6 +101,27 @@ def test_batched_fast_within_1e12(nl, orc):
         assert _rel(xa, xr) <= 1e-12, (j, n, _rel(xa, xr))
 
 
+@pytest.mark.parametrize("nrhs", [0, 3])
+def test_batched_zero_sizes_and_factor_only(nl, orc, nrhs):
+    """Members of size 0 (an empty DMatrix: Cholesky::new succeeds, nothing to
+    store) mixed into a batch, with and without right-hand sides."""
+    ns = [0, 5, 0, 64, 1, 0, 33, 8, 0]
+    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 21, nl.EXACT)
+    Ar, Br = A.copy(), B.copy()
+    info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
+    assert np.array_equal(info, info_r) and not info.any()
+    assert np.array_equal(_bits(Ag), _bits(Ar)), "factor bits differ"
+    assert np.array_equal(_bits(Bg), _bits(Br)), "solution bits differ"
+
+
+def test_batched_empty(nl):
+    bt = nl.CholeskyBatch([], 4)
+    A, B = bt.alloc()
+    info = bt.solve(A, B, nl.EXACT)
+    torch.cuda.synchronize()
+    assert info.numel() == 0
+
+
 def test_batched_not_positive_definite(nl, orc):
     ns = [5, 17, 40, 64, 3]
     A, offA, B, offB = gen.spd_batch(13, ns, 4)

@@ -19,7 +19,8 @@ def _bits(a):
     return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
 
 
-@pytest.mark.parametrize("shape", [(5, 3), (3, 5), (4, 4), (40, 12), (300, 64), (64, 64), (129, 70)])
+@pytest.mark.parametrize("shape", [(5, 3), (3, 5), (4, 4), (40, 12), (300, 64), (64, 64), (129, 70),
+                                   (1, 1), (1, 5), (5, 1), (600, 2), (2, 600)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_qr_exact_bitwise(nl, orc, shape, dtype):
     A = gen.uniform(300 + shape[0] * 7 + shape[1], shape).astype(dtype)
