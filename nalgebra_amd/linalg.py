@@ -291,9 +291,13 @@ class QR:
         check(fn.__name__, st, h)
 
     def q_tr_mul(self, b: DMatrix) -> None:
-        """QR::q_tr_mul (qr.rs:157-171), in place."""
+        """QR::q_tr_mul (qr.rs:157-171), in place.  A b whose row count is not
+        the decomposed matrix's panics there, in the reflection's dot product
+        (reflection.rs:80, blas.rs:32-35)."""
         if self.qr.dtype != torch.float64:
             raise TypeError("q_tr_mul: float64 only")
+        if b.nrows != self.qr.nrows:
+            raise ValueError("Dot product dimensions mismatch: left rows != right rows.")
         h = handle()
         st = lib().nrocm_dqr_qtmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), self.qr.nrows,
                                    _ptr(self.diag), b.ncols, _ptr(b.data), b.nrows)

@@ -157,3 +157,46 @@ def test_handle_still_works_after_errors(env, nl):
     x = nl.DMatrix.from_numpy(np.array([[1.0], [2.0]]))
     assert lu.solve_mut(x)
     assert np.allclose(x.numpy()[:, 0], np.linalg.solve(A, [1.0, 2.0]), rtol=1e-14)
+
+
+def test_host_layer_panics_match_nalgebra(nl):
+    """The Python host layer raises where nalgebra panics, with nalgebra's
+    message, before anything reaches the device."""
+    rng = np.random.default_rng(3)
+    sq = nl.DMatrix.from_numpy(rng.standard_normal((4, 4)) + 4 * np.eye(4))
+    tall = nl.DMatrix.from_numpy(rng.standard_normal((4, 3)))
+    b3 = nl.DMatrix.from_numpy(np.ones((3, 1)))
+    b4 = nl.DMatrix.from_numpy(np.ones((4, 1)))
+    b5 = nl.DMatrix.from_numpy(np.ones((5, 1)))
+    # lu.rs:248-255
+    with pytest.raises(ValueError, match="LU solve matrix dimension mismatch."):
+        sq.lu().solve_mut(b3)
+    with pytest.raises(ValueError, match="LU solve: unable to solve a non-square system."):
+        tall.lu().solve_mut(b4)
+    with pytest.raises(ValueError, match="non-square"):
+        tall.lu().determinant()
+    # cholesky.rs:222
+    with pytest.raises(ValueError, match="The input matrix must be square."):
+        tall.cholesky()
+    spd = nl.DMatrix.from_numpy(np.eye(4) * 2.0).cholesky()
+    with pytest.raises(ValueError, match="dimension mismatch"):
+        spd.solve_mut(b3)
+    # qr.rs:210-217; q_tr_mul panics in the reflection's dot product (blas.rs:32-35)
+    with pytest.raises(ValueError, match="QR solve matrix dimension mismatch."):
+        sq.qr().solve_mut(b5)
+    with pytest.raises(ValueError, match="QR solve: unable to solve a non-square system."):
+        tall.qr().solve_mut(b4)
+    with pytest.raises(ValueError, match="Dot product dimensions mismatch"):
+        sq.qr().q_tr_mul(b5)
+    # blas.rs:788-796
+    c = nl.DMatrix.zeros(4, 1)
+    with pytest.raises(ValueError, match="gemm: dimensions mismatch for multiplication."):
+        c.gemm(1.0, sq, b3, 0.0)
+    with pytest.raises(ValueError, match="gemm: dimensions mismatch for addition."):
+        c.gemm(1.0, tall, nl.DMatrix.from_numpy(np.ones((3, 2))), 0.0)
+    with pytest.raises(ValueError):
+        nl.solve_mixed_ir(tall, b4)
+    # none of the above left the handle in an error state
+    x = nl.DMatrix.from_numpy(np.ones((4, 1)))
+    assert sq.lu().solve_mut(x)
+    assert np.allclose(sq.numpy() @ x.numpy(), 1.0, rtol=1e-13)
