@@ -208,6 +208,39 @@ def test_adjoint_exact_matches_oracle(nl, orc):
         assert np.array_equal(dAg[offA[j]:offA[j] + n * n].reshape(n, n, order="F"), dAr)
 
 
+def test_adjoint_several_rhs_and_zero_sizes(nl, orc):
+    """nrhs = 6 spans two register chunks of 4: dB is the per-column adjoint and
+    dA sums the per-column terms in column order, ((dA_0 + dA_1) + ...) + dA_5.
+    Size-0 members are skipped by every load and store."""
+    ns = [0, 7, 0, 33, 64, 1, 12, 0, 40]
+    nrhs = 6
+    A, offA, B, offB = gen.spd_batch(41, ns, nrhs)
+    bt = nl.CholeskyBatch(ns, nrhs)
+    Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+    info = bt.solve(Ad, Bd, nl.EXACT)
+    dXh = gen.uniform(42, (B.size,)).ravel()
+    dA, dB = bt.backward(Ad, Bd, torch.from_numpy(dXh).cuda())
+    torch.cuda.synchronize()
+    assert not info.cpu().numpy().any()
+    Lg = Ad.cpu().numpy(); Xg = Bd.cpu().numpy(); dAg = dA.cpu().numpy(); dBg = dB.cpu().numpy()
+    for j, n in enumerate(ns):
+        if n == 0:
+            continue
+        L = Lg[offA[j]:offA[j] + n * n].reshape(n, n, order="F")
+        X = Xg[offB[j]:offB[j] + n * nrhs].reshape(n, nrhs, order="F")
+        G = dXh[offB[j]:offB[j] + n * nrhs].reshape(n, nrhs, order="F")
+        dA_ref = None
+        dB_ref = np.zeros((n, nrhs), order="F")
+        for q in range(nrhs):
+            dAq, dBq = orc.potrs_adjoint(L, X[:, q], G[:, q])
+            dB_ref[:, q] = dBq
+            dA_ref = dAq if dA_ref is None else dA_ref + dAq
+        got_dB = dBg[offB[j]:offB[j] + n * nrhs].reshape(n, nrhs, order="F")
+        got_dA = dAg[offA[j]:offA[j] + n * n].reshape(n, n, order="F")
+        assert np.array_equal(_bits(got_dB), _bits(dB_ref)), (j, n)
+        assert np.array_equal(_bits(got_dA), _bits(dA_ref)), (j, n)
+
+
 def test_adjoint_central_differences(nl):
     n = 32
     G = gen.normal(41, (n, n)); A = G @ G.T + n * np.eye(n); b = gen.normal(42, (n,))
