@@ -136,6 +136,8 @@ def test_batch_argument_errors(env):
     assert L.nrocm_sgels_batched(h, bt, 0, d, d, d, 1, i) == -3
     assert L.nrocm_sgels_batched(h, bt, 65, d, d, d, 1, i) == -3
     assert L.nrocm_sgels_batched(h, bt, 2, d, d, d, -1, i) == -7
+    # a system with fewer rows than unknowns makes the batch invalid for that n
+    assert L.nrocm_sgels_batched(h, bt, 4, d, d, d, 1, i) == -2
     lanes = (C.c_int * 8)()
     assert L.nrocm_batch_set_schedule(bt, 9, lanes) == -2
     assert L.nrocm_batch_set_schedule(bt, EXACT, None) == -3
@@ -146,6 +148,15 @@ def test_batch_argument_errors(env):
     assert L.nrocm_batch_set_schedule(bt, EXACT, lanes) == -3
     assert L.nrocm_batch_schedule(bt, EXACT, None) == -3
     assert L.nrocm_batch_autotune(h, bt, 2, lanes) == -3
+    assert L.nrocm_batch_destroy(bt) == 0
+    # sizes outside what the batched kernels hold: refused before any launch
+    big = np.array([65, 3000], dtype=np.int32)
+    assert L.nrocm_batch_create(h, 2, big.ctypes.data_as(C.POINTER(C.c_int)), 1, C.byref(bp)) == 0
+    bt = bp.value
+    assert L.nrocm_dpotrf_potrs_batched(h, bt, d, d, i, EXACT) == 3       # NROCM_ERR_UNSUPPORTED
+    assert L.nrocm_dpotrs_backward_batched(h, bt, d, d, d, d, d) == 3
+    assert L.nrocm_sgels_batched(h, bt, 4, d, d, d, 1, i) == 3
+    assert b"64" in L.nrocm_last_error(h) or b"2048" in L.nrocm_last_error(h)
     assert L.nrocm_batch_destroy(bt) == 0
 
 
