@@ -101,6 +101,30 @@ def test_batched_least_squares(nl, n):
     assert worst1 < worst0
 
 
+@pytest.mark.parametrize("n", [64, 24])
+def test_batched_least_squares_square(nl, n):
+    """m_j = n, the fewest rows allowed: the last reflector has an empty tail
+    (tau = 0, the identity) and the least-squares solution is the square solve."""
+    ms, As, bs = _ls_batch(700, [n] * 4 + [n + 1, n + 3], n)
+    bt = nl.LeastSquaresBatch(ms, n)
+    Ad = torch.from_numpy(np.concatenate([a.ravel(order="F") for a in As])).cuda()
+    bd = torch.from_numpy(np.concatenate(bs)).cuda()
+    x0, info0 = bt.solve(Ad, bd, refine=0)
+    x1, info1 = bt.solve(Ad, bd, refine=1)
+    torch.cuda.synchronize()
+    assert not info0.cpu().numpy().any() and not info1.cpu().numpy().any()
+    x0 = x0.cpu().numpy(); x1 = x1.cpu().numpy()
+    for j in range(len(ms)):
+        A = As[j].astype(np.float64); b = bs[j].astype(np.float64)
+        xr = np.linalg.lstsq(A, b, rcond=None)[0]
+        kappa = np.linalg.cond(A)
+        e0 = np.linalg.norm(x0[j] - xr) / np.linalg.norm(xr)
+        e1 = np.linalg.norm(x1[j] - xr) / np.linalg.norm(xr)
+        print(f"m={ms[j]} n={n} kappa={kappa:.0f}: {e0:.2e} -> {e1:.2e}")
+        assert e0 <= 1e-5 * kappa, (j, e0, kappa)       # FP32 Householder QR
+        assert e1 <= 1e-6 and e1 <= max(e0, 1e-12), (j, e1)
+
+
 def test_batched_least_squares_rank_deficient(nl):
     n = 16
     ms, As, bs = _ls_batch(600, [100, 120], n)
