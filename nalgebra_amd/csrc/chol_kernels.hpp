@@ -118,9 +118,9 @@ __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch,
         // The multipliers are read in chunks of UC, one chunk ahead of its
         // updates; the scheduling barriers stop the compiler from hoisting
         // every read of the step (which would need 2*(NB-k) more VGPRs).
-        // Smaller chunks at NB = 32, 40 and 64 leave room for one more
+        // Smaller chunks at NB = 32, 40, 48 and 64 leave room for one more
         // wavefront per SIMD.
-        constexpr int UC = (NB == 64 || NB == 40 || NB == 32) ? 4 : 8;
+        constexpr int UC = (NB == 64 || NB == 48 || NB == 40 || NB == 32) ? 4 : 8;
         constexpr int NC = (NB + UC - 1) / UC;
         const double *xg = xch + g * G;
         const int cfirst = (k + 1) / UC;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(64) void potrs_kernel(
 // for the substitutions, so L is written once and never read back.  Members
 // whose factorisation failed keep their right-hand sides.
 template <int NB, int G, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NB >= 56 ? 3 : (NB == 40 || NB == 32) ? 4 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NB >= 56 ? 3 : (NB == 48 || NB == 40 || NB == 32) ? 4 : 1)))
 void potrf_potrs_kernel(const int *__restrict__ order, int begin, int end,
                                                          const int *__restrict__ nsz,
                                                          double *const *__restrict__ Aptr,
