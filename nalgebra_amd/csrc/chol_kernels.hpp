@@ -118,9 +118,9 @@ __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch,
         // The multipliers are read in chunks of UC, one chunk ahead of its
         // updates; the scheduling barriers stop the compiler from hoisting
         // every read of the step (which would need 2*(NB-k) more VGPRs).
-        // Smaller chunks at NB = 32, 40, 48 and 64 leave room for one more
-        // wavefront per SIMD.
-        constexpr int UC = (NB == 64 || NB == 48 || NB == 40 || NB == 32) ? 4 : 8;
+        // Smaller chunks at NB = 32..64 leave room for one more wavefront
+        // per SIMD (NB = 40, 48) or avoid spills (NB = 56, 64).
+        constexpr int UC = (NB == 64 || NB == 56 || NB == 48 || NB == 40 || NB == 32) ? 4 : 8;
         constexpr int NC = (NB + UC - 1) / UC;
         const double *xg = xch + g * G;
         const int cfirst = (k + 1) / UC;
