@@ -198,7 +198,12 @@ int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *co
  * was not reached.  With NROCM_PREC_WINDOW set, each 64-column block is factored
  * with its pivots chosen among the 128 rows starting at the block's diagonal
  * (one workgroup, no full-height pivot search); the rows below that window
- * are eliminated by a triangular solve with the block's U.  With
+ * are eliminated by a triangular solve with the block's U.  When those
+ * factors are unusable (a zero pivot inside a window, a non-finite backward
+ * error, or a refinement step from the third on that does not halve the
+ * backward error, or tol missed after maxit solves) the call factors A again
+ * with partial pivoting and refines from x = 0; iters_host then counts the
+ * solves of both attempts and nrocm_ir_fallback reports it.  With
  * NROCM_PREC_GMRES set, each refinement cycle is FGMRES(20) on
  * A x = b in FP64, right-preconditioned by the factors (Carson & Higham 2017),
  * and iters_host counts preconditioner solves.  No reference counterpart:
@@ -207,6 +212,10 @@ int nrocm_sgels_batched(nrocm_handle handle, nrocm_batch batch, int n, float *co
 int nrocm_dgesv_ir(nrocm_handle handle, int n, const double *a, int lda, const double *b,
                    double *x, int precision, double tol, int maxit, int *iters_host,
                    double *berr_host);
+/* 1 in *fell_back_host when the handle's last nrocm_dgesv_ir call replaced
+ * windowed factors by partial-pivoting factors, else 0.  No reference
+ * counterpart. */
+int nrocm_ir_fallback(nrocm_handle handle, int *fell_back_host);
 
 /* ---------------------------------------------------------------- synthetic inputs
  * Host-side seeded generators (xoshiro256** seeded through SplitMix64) so

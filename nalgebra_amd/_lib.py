@@ -64,6 +64,7 @@ _PROTOS = {
     "nrocm_dqr_rsolve": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp]),
     "nrocm_sgels_batched": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _vp]),
     "nrocm_dgesv_ir": (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _d, _i, _ip, _dp]),
+    "nrocm_ir_fallback": (_i, [_vp, _ip]),
     "nrocm_gen_uniform": (None, [C.c_uint64, _sz, _d, _d, _vp]),
     "nrocm_gen_normal": (None, [C.c_uint64, _sz, _vp]),
     "nrocm_gen_spd_batch": (None, [C.c_uint64, _i, _ip, _i, _vp, _vp, _vp, _vp]),

@@ -213,7 +213,8 @@ void report_lanes(nrocm_batch bt, int mi, int *lanes)
 {
     if (!lanes) return;
     for (int c = 0; c < 8; ++c) lanes[c] = 0;
-    for (size_t bi = 0; bi < bt->bins.size(); ++bi) lanes[bt->bins[bi].kernel] = bt->lanes[mi][bi];
+    for (size_t bi = 0; bi < bt->bins.size(); ++bi)
+        if (bt->bins[bi].kernel < 8) lanes[bt->bins[bi].kernel] = bt->lanes[mi][bi];
 }
 
 }  // namespace
@@ -326,6 +327,8 @@ int nrocm_batch_set_schedule(nrocm_batch bt, int mode, const int *lanes)
     if (!bt) return -1;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -2;
     if (!lanes) return -3;
+    if (bt->max_n > 64)
+        return nrocm_fail(bt->handle, NROCM_ERR_UNSUPPORTED, "batched Cholesky supports n <= 64");
     for (int c = 0; c < 8; ++c) {
         const int G = lanes[c];
         if (G != 0 && ((G != 8 && G != 16 && G != 32 && G != 64) || G < 8 * (c + 1))) return -3;
@@ -342,6 +345,8 @@ int nrocm_batch_schedule(nrocm_batch bt, int mode, int *lanes)
     if (!bt) return -1;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -2;
     if (!lanes) return -3;
+    if (bt->max_n > 64)
+        return nrocm_fail(bt->handle, NROCM_ERR_UNSUPPORTED, "batched Cholesky supports n <= 64");
     report_lanes(bt, mode == NROCM_EXACT ? 0 : 1, lanes);
     return NROCM_OK;
 }

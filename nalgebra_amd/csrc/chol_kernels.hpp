@@ -779,6 +779,9 @@ __global__ __launch_bounds__(64) void potrs_backward_kernel(
         }
         lds_barrier();
     }
+    // no right-hand sides: X = A^{-1} B is empty and so is its gradient
+    if (nrhs == 0 && r < n)
+        for (int c = 0; c < n; ++c) dA[r + (size_t)c * n] = 0.0;
 }
 
 // Launches the fused kernel for members [begin, end) of `order` with G lanes

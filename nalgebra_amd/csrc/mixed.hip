@@ -554,6 +554,11 @@ int fgmres(nrocm_handle h, int n, const double *A, int lda, const double *b, dou
     return berr <= tol ? NROCM_OK : NROCM_ERR_NOT_CONVERGED;
 }
 
+// Status a windowed attempt returns when its factors are unusable: a zero
+// pivot inside a window, a non-finite backward error, or refinement that
+// stops contracting.  The caller then factors again with partial pivoting.
+constexpr int IR_RETRY_PARTIAL = -1000;
+
 template <typename T>
 int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, double *x, NrocmTrailing trail,
             bool window, bool gmres, double tol, int maxit, int *iters, double *berr_out)
@@ -629,8 +634,12 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
     int it = 0;
     unsigned long long hred[4];
     int hinfo[2];
-    if (gmres) return fgmres(h, n, A, lda, b, x, W, d, perm, InvL, InvU, r, part, sync, tagged, err, red, info, Vk, Zk,
-                             hdev, tol, maxit, iters, berr_out);
+    if (gmres) {
+        s = fgmres(h, n, A, lda, b, x, W, d, perm, InvL, InvU, r, part, sync, tagged, err, red, info, Vk, Zk, hdev,
+                   tol, maxit, iters, berr_out);
+        return (window && s == NROCM_ERR_NOT_CONVERGED) ? IR_RETRY_PARTIAL : s;
+    }
+    double prev = INFINITY;
     for (it = 1; it <= maxit; ++it) {
         hipLaunchKernelGGL((gather_kernel<T>), dim3(nrocm_cdiv(n, 256)), dim3(256), 0, st, r, perm, d, n);
         IR_TRY(hipGetLastError());
@@ -652,10 +661,19 @@ int gesv_ir(nrocm_handle h, int n, const double *A, int lda, const double *b, do
             std::memcpy(&v[q], &bits, sizeof(double));
         }
         berr = v[2] / (v[0] * v[3] + v[1]);
+        // windowed factors that hit a zero pivot, give a non-finite solution or
+        // stop contracting the error (a step that does not halve it, from the
+        // third step on) are replaced by partial-pivoting factors
+        if (window && (hinfo[0] != 0 || !std::isfinite(berr) || (it >= 3 && berr > tol && berr > 0.5 * prev))) {
+            if (iters) *iters = it;
+            return IR_RETRY_PARTIAL;
+        }
         if (!(berr > tol)) break;  // also stops on NaN
+        prev = berr;
     }
     if (it > maxit) it = maxit;
     if (iters) *iters = it;
+    if (window && berr > tol) return IR_RETRY_PARTIAL;
     if (berr_out) *berr_out = berr;
     if (hinfo[0] != 0) return nrocm_fail(h, NROCM_ERR_NOT_CONVERGED, "singular factorization");
     return berr <= tol ? NROCM_OK : NROCM_ERR_NOT_CONVERGED;
@@ -685,12 +703,38 @@ int nrocm_dgesv_ir(nrocm_handle h, int n, const double *a, int lda, const double
         if (berr_host) *berr_host = 0.0;
         return NROCM_OK;
     }
-    if (precision == NROCM_PREC_FP64)
-        return gesv_ir<double>(h, n, a, lda, b, x, TRAIL_F64, window, gmres, tol, maxit, iters_host, berr_host);
-    const NrocmTrailing tr = precision == NROCM_PREC_BF16   ? TRAIL_BF16
+    const NrocmTrailing tr = precision == NROCM_PREC_FP64     ? TRAIL_F64
+                             : precision == NROCM_PREC_BF16   ? TRAIL_BF16
                              : precision == NROCM_PREC_BF16X3 ? TRAIL_BF16X3
                                                               : TRAIL_F32;
-    return gesv_ir<float>(h, n, a, lda, b, x, tr, window, gmres, tol, maxit, iters_host, berr_host);
+    auto attempt = [&](bool win, int *it_out, double *be_out) {
+        return precision == NROCM_PREC_FP64
+                   ? gesv_ir<double>(h, n, a, lda, b, x, tr, win, gmres, tol, maxit, it_out, be_out)
+                   : gesv_ir<float>(h, n, a, lda, b, x, tr, win, gmres, tol, maxit, it_out, be_out);
+    };
+    h->ir_fallback = 0;
+    int it1 = 0;
+    double be1 = INFINITY;
+    int s = attempt(window, &it1, &be1);
+    if (s == IR_RETRY_PARTIAL) {
+        // the windowed pivots were not good enough for this matrix: factor
+        // again with partial pivoting over all remaining rows
+        h->ir_fallback = 1;
+        int it2 = 0;
+        s = attempt(false, &it2, &be1);
+        it1 += it2;
+    }
+    if (iters_host) *iters_host = it1;
+    if (berr_host) *berr_host = be1;
+    return s;
+}
+
+int nrocm_ir_fallback(nrocm_handle h, int *fell_back_host)
+{
+    if (!h) return -1;
+    if (!fell_back_host) return -2;
+    *fell_back_host = h->ir_fallback;
+    return NROCM_OK;
 }
 
 }  // extern "C"

@@ -57,6 +57,9 @@ struct nrocm_handle_s {
     // Epoch of the last look-back triangular solve launched on this handle
     // (tags its published values; never reused while the handle lives).
     unsigned trsv_epoch = 0;
+    // 1 when the last nrocm_dgesv_ir call replaced its windowed factors by
+    // partial-pivoting factors.
+    int ir_fallback = 0;
 };
 
 // Runs `body(stream)` through the handle's graph cache: on a miss the launches

@@ -190,8 +190,8 @@ class LU:
             raise ValueError("LU solve: unable to solve a non-square system.")
         info = torch.zeros(1, dtype=torch.int32, device=b.data.device)
         h = handle()
-        st = lib().nrocm_dgetrs(h, self.lu.nrows, b.ncols, _ptr(self.lu.data), self.lu.nrows,
-                                _ptr(self.ipiv), _ptr(b.data), b.nrows, _ptr(info), self.mode)
+        st = lib().nrocm_dgetrs(h, self.lu.nrows, b.ncols, _ptr(self.lu.data), max(self.lu.nrows, 1),
+                                _ptr(self.ipiv), _ptr(b.data), max(b.nrows, 1), _ptr(info), self.mode)
         check("nrocm_dgetrs", st, h)
         return int(info.item()) == 0
 
@@ -257,8 +257,8 @@ class Cholesky:
         if b.nrows != self.chol.nrows:
             raise ValueError("Cholesky solve: dimension mismatch")
         h = handle()
-        st = lib().nrocm_dpotrs(h, self.chol.nrows, b.ncols, _ptr(self.chol.data), self.chol.nrows,
-                                _ptr(b.data), b.nrows, self.mode)
+        st = lib().nrocm_dpotrs(h, self.chol.nrows, b.ncols, _ptr(self.chol.data), max(self.chol.nrows, 1),
+                                _ptr(b.data), max(b.nrows, 1), self.mode)
         check("nrocm_dpotrs", st, h)
 
     def solve(self, b: DMatrix) -> DMatrix:
@@ -299,8 +299,8 @@ class QR:
         if b.nrows != self.qr.nrows:
             raise ValueError("Dot product dimensions mismatch: left rows != right rows.")
         h = handle()
-        st = lib().nrocm_dqr_qtmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), self.qr.nrows,
-                                   _ptr(self.diag), b.ncols, _ptr(b.data), b.nrows)
+        st = lib().nrocm_dqr_qtmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), max(self.qr.nrows, 1),
+                                   _ptr(self.diag), b.ncols, _ptr(b.data), max(b.nrows, 1))
         check("nrocm_dqr_qtmul", st, h)
 
     def solve_mut(self, b: DMatrix) -> bool:
@@ -315,8 +315,8 @@ class QR:
     def _rsolve(self, b: DMatrix) -> bool:
         info = torch.zeros(1, dtype=torch.int32, device=b.data.device)
         h = handle()
-        st = lib().nrocm_dqr_rsolve(h, self.qr.ncols, _ptr(self.qr.data), self.qr.nrows,
-                                    _ptr(self.diag), b.ncols, _ptr(b.data), b.nrows, _ptr(info))
+        st = lib().nrocm_dqr_rsolve(h, self.qr.ncols, _ptr(self.qr.data), max(self.qr.nrows, 1),
+                                    _ptr(self.diag), b.ncols, _ptr(b.data), max(b.nrows, 1), _ptr(info))
         check("nrocm_dqr_rsolve", st, h)
         return int(info.item()) == 0
 
@@ -345,8 +345,16 @@ def solve_mixed_ir(a: DMatrix, b: DMatrix, precision: int = _lib.PREC_FP32, tol:
     iters = C.c_int(0)
     berr = C.c_double(0.0)
     h = handle()
-    st = lib().nrocm_dgesv_ir(h, a.nrows, _ptr(a.data), a.nrows, _ptr(b.data), _ptr(x.data), precision,
+    st = lib().nrocm_dgesv_ir(h, a.nrows, _ptr(a.data), max(a.nrows, 1), _ptr(b.data), _ptr(x.data), precision,
                               tol, maxit, C.byref(iters), C.byref(berr))
     if st not in (_lib.OK, _lib.ERR_NOT_CONVERGED):
         check("nrocm_dgesv_ir", st, h)
     return x, iters.value, berr.value, st == _lib.OK
+
+
+def last_ir_fallback() -> bool:
+    """True when the last solve_mixed_ir call with PREC_WINDOW replaced its
+    windowed factors by partial-pivoting factors."""
+    fb = C.c_int(0)
+    check("nrocm_ir_fallback", lib().nrocm_ir_fallback(handle(), C.byref(fb)))
+    return bool(fb.value)

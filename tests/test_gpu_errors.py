@@ -157,6 +157,11 @@ def test_batch_argument_errors(env):
     assert L.nrocm_dpotrs_backward_batched(h, bt, d, d, d, d, d) == 3
     assert L.nrocm_sgels_batched(h, bt, 4, d, d, d, 1, i) == 3
     assert b"64" in L.nrocm_last_error(h) or b"2048" in L.nrocm_last_error(h)
+    # the schedule calls hold 8 size classes (n <= 64): nothing is read or written
+    guard = (C.c_int * 16)(*([-7] * 16))
+    assert L.nrocm_batch_schedule(bt, EXACT, guard) == 3
+    assert L.nrocm_batch_set_schedule(bt, EXACT, guard) == 3
+    assert list(guard) == [-7] * 16
     assert L.nrocm_batch_destroy(bt) == 0
 
 

@@ -167,3 +167,64 @@ def test_mixed_ir_well_conditioned_bf16(nl):
                                          nl.PREC_BF16, tol=1e-14, maxit=40)
     print(f"bf16 kappa=10: iters={iters} berr={be:.3e}")
     assert ok and berr(A, x.numpy()[:, 0], b) <= 2e-14
+
+
+def _window_trap(kind, n):
+    """Nonsingular, well-conditioned matrices on which pivots searched only in
+    the next 128 rows fail: their factors are singular or unstable, so the
+    solver has to factor again with partial pivoting."""
+    rng = np.random.default_rng({"anti": 11, "swap": 12, "rankdef": 13}[kind])
+    if kind == "anti":
+        # every column's only nonzero lies in the mirrored row, below the window
+        return np.fliplr(np.eye(n)) * rng.uniform(1.0, 2.0, n)
+    if kind == "swap":
+        # [[0, B], [C, 0]]: the first half of the columns has nothing in the
+        # first half of the rows, so the first windows are exactly singular
+        h = n // 2
+        A = np.zeros((n, n))
+        A[:h, h:] = haar(401, h)
+        A[h:, :h] = haar(402, h)
+        return A
+    # the window rows restricted to the block's 64 columns are rank 63 up to
+    # 1e-12, while the rows below and the other columns are O(1): the window's
+    # last pivot is ~1e-12 and L21 = A21 U11^{-1} ~ 1e12 (growth ~ 1e12);
+    # partial pivoting takes that pivot from the rows below instead
+    A = rng.standard_normal((n, n))
+    X = rng.standard_normal((128, 63))
+    Y = rng.standard_normal((64, 63))
+    A[:128, :64] = X @ Y.T / np.sqrt(63) + 1e-12 * rng.standard_normal((128, 64))
+    return A
+
+
+@pytest.mark.parametrize("kind", ["anti", "swap", "rankdef"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_mixed_ir_window_falls_back(nl, kind, prec):
+    """Windowed pivoting is not partial pivoting: where its factors are
+    singular or unstable the call must still converge to the 1e-14 bar, by
+    factoring again with partial pivoting (lu.rs:104 searches every remaining
+    row, min_max.rs:221-241)."""
+    n = 512
+    A = _window_trap(kind, n)
+    b = np.random.default_rng(7).standard_normal(n)
+    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32, "bf16x3": nl.PREC_BF16X3}[prec] | nl.PREC_WINDOW
+    print(f"window trap {kind} {prec}", flush=True)
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
+                                         tol=1e-14, maxit=30)
+    fb = nl.last_ir_fallback()
+    xv = x.numpy()[:, 0]
+    print(f"  iters={iters} berr={be:.3e} ok={ok} fallback={fb}")
+    assert ok and be <= 1e-14
+    assert berr(A, xv, b) <= 2e-14
+    assert np.allclose(xv, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+    if kind in ("anti", "swap"):
+        assert fb, "a singular window must trigger the partial-pivoting factorisation"
+
+
+def test_mixed_ir_no_fallback_on_haar(nl):
+    """The kappa = 1e4 Haar matrices keep their windowed factors."""
+    n = 512
+    A = kappa_matrix(470, n)
+    b = gen.normal(471, (n,))
+    x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)),
+                                         nl.PREC_FP32 | nl.PREC_WINDOW, tol=1e-14, maxit=30)
+    assert ok and not nl.last_ir_fallback()
