@@ -95,7 +95,9 @@ int nrocm_dgetrf(nrocm_handle handle, int m, int n, double *a, int lda, int *ipi
 
 /* LU::solve_mut (lu.rs:242-260): B (n x nrhs) <- A^{-1} B.  info[0] = 0, or
  * 1 + the index of an exact zero on U's diagonal (solve_mut returns false and
- * B holds garbage).  Shape of LUScalar::xgetrs. */
+ * B holds garbage); the n >= 1024 path also reports -1 if an internal wait
+ * timed out and -2 for a pivot outside [0, n) (not applied).  Any n (n >= 1024: one 64-row block per workgroup, pipelined).
+ * Shape of LUScalar::xgetrs. */
 int nrocm_dgetrs(nrocm_handle handle, int n, int nrhs, const double *lu, int lda,
                  const int *ipiv, double *b, int ldb, int *info, int mode);
 

@@ -36,3 +36,10 @@ int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, in
 // chol_batched.hip: n <= 64 single-matrix path.
 int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int ldb, int nrhs, int *info,
                       bool exact);
+
+// solve_lookback.hip: LU::solve_mut for any n, one 64-row block per workgroup.
+// Scratch: nrocm_getrs_scratch_bytes(n) bytes of device memory; allocates nothing.
+constexpr int NROCM_GETRS_LOOKBACK_MIN = 1024;  // nrocm_dgetrs uses it from this n on
+size_t nrocm_getrs_scratch_bytes(int n);
+int nrocm_getrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const double *lu, int lda, const int *ipiv,
+                         double *b, int ldb, int *info, bool exact, void *scratch);

@@ -503,8 +503,13 @@ int nrocm_dgetrs(nrocm_handle h, int n, int nrhs, const double *lu, int lda, con
     if (ldb < std::max(1, n)) return -8;
     if (!info) return -9;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -10;
-    if ((size_t)n * sizeof(double) > 60 * 1024)
-        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "getrs: n > 7680 (use nrocm_dgesv_ir)");
+    if (n >= NROCM_GETRS_LOOKBACK_MIN) {
+        // one workgroup per 64-row block, pipelined through published blocks
+        void *ws = nullptr;
+        const int s = nrocm_workspace(h, nrocm_getrs_scratch_bytes(n), &ws);
+        if (s != NROCM_OK) return s;
+        return nrocm_getrs_lookback(h, h->stream, n, nrhs, lu, lda, ipiv, b, ldb, info, mode == NROCM_EXACT, ws);
+    }
     hipLaunchKernelGGL((getrs_kernel<double>), dim3(1), dim3(1024), (size_t)std::max(n, 1) * sizeof(double),
                        h->stream, lu, lda, n, ipiv, b, ldb, nrhs, info, mode == NROCM_EXACT ? 1 : 0);
     NROCM_CHECK_LAUNCH(h);
