@@ -101,6 +101,20 @@ int nrocm_dgetrf(nrocm_handle handle, int m, int n, double *a, int lda, int *ipi
 int nrocm_dgetrs(nrocm_handle handle, int n, int nrhs, const double *lu, int lda,
                  const int *ipiv, double *b, int ldb, int *info, int mode);
 
+/* LU::solve_mut as nrocm_dgetrs, allocating nothing: no device allocation, no
+ * graph, only kernel launches on the handle's stream, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  Its scratch is
+ * a static device array of the library, one per GPU, held until the process
+ * ends; n above NROCM_NOALLOC_MAX_N returns NROCM_ERR_UNSUPPORTED.  Calls
+ * on the same GPU share that array: calls that overlap in time on one GPU
+ * (two streams, two threads) race and must be serialised by the caller.
+ * Same arguments, results and info as nrocm_dgetrs (info = -1: an internal
+ * wait timed out). */
+#define NROCM_NOALLOC_MAX_N 16384
+int nrocm_dgetrs_noalloc(nrocm_handle handle, int n, int nrhs, const double *lu, int lda,
+                         const int *ipiv, double *b, int ldb, int *info, int mode);
+
 /* ---------------------------------------------------------------- Cholesky (a8-a9)
  * Cholesky::new (src/linalg/cholesky.rs:196-198,221-272): reads the lower
  * triangle, writes L there, leaves the strict upper triangle untouched.

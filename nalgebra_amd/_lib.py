@@ -48,6 +48,7 @@ _PROTOS = {
     "nrocm_sgemm": (_i, [_vp, _sz, _sz, _sz, _f, _vp, _pd, _pd, _vp, _pd, _pd, _f, _vp, _pd, _pd]),
     "nrocm_dgetrf": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i]),
     "nrocm_dgetrs": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i]),
+    "nrocm_dgetrs_noalloc": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i]),
     "nrocm_dpotrf": (_i, [_vp, _i, _vp, _i, _vp, _i]),
     "nrocm_dpotrs": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _i]),
     "nrocm_batch_create": (_i, [_vp, _i, _ip, _i, C.POINTER(_vp)]),

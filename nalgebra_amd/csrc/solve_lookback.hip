@@ -20,6 +20,7 @@
 #include "nrocm_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -286,4 +287,71 @@ int nrocm_getrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const 
         NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;
+}
+
+// ---------------------------------------------------------------- the allocation-free solve
+// nrocm_dgetrs_noalloc keeps the solve's scratch (ticket, permutation, tags)
+// in this file-scope device array instead of the handle's workspace: each GPU
+// the library runs on holds one copy, NROCM_NOALLOC_SCRATCH_BYTES bytes, until
+// the process ends.  Host code never passes the array's name to a kernel (on
+// the host it is a stand-in, not a device address): the device address is
+// looked up once per device and kept in scratch_dev.
+namespace {
+
+constexpr size_t NROCM_NOALLOC_SCRATCH_BYTES =
+    256 + (((size_t)NROCM_NOALLOC_MAX_N * sizeof(int) + 255) & ~(size_t)255) +
+    2 * (size_t)NROCM_NOALLOC_MAX_N * sizeof(unsigned long long);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NROCM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the handle's device (looked up on the first
+// call for that device, with the device current).
+int scratch_ptr(nrocm_handle h, void **out)
+{
+    if (h->device < 0 || h->device >= kMaxDevices) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "device index >= 64");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[h->device]) {
+        int cur = 0;
+        NROCM_CHECK_HIP(h, hipGetDevice(&cur));
+        if (cur != h->device) NROCM_CHECK_HIP(h, hipSetDevice(h->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != h->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return nrocm_fail_hip(h, e, "hipGetSymbolAddress(scratch)");
+        scratch_dev[h->device] = p;
+    }
+    *out = scratch_dev[h->device];
+    return NROCM_OK;
+}
+
+}  // namespace
+
+extern "C" int nrocm_dgetrs_noalloc(nrocm_handle h, int n, int nrhs, const double *lu, int lda, const int *ipiv,
+                                    double *b, int ldb, int *info, int mode)
+{
+    if (!h) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (!lu && n > 0) return -4;
+    if (lda < std::max(1, n)) return -5;
+    if (!ipiv && n > 0) return -6;
+    if (!b && n > 0 && nrhs > 0) return -7;
+    if (ldb < std::max(1, n)) return -8;
+    if (!info) return -9;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -10;
+    if (n > NROCM_NOALLOC_MAX_N)
+        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED,
+                          "getrs_noalloc: n exceeds NROCM_NOALLOC_MAX_N (16384), the rows its static scratch holds");
+    void *ptr = nullptr;
+    const int s = scratch_ptr(h, &ptr);
+    if (s != NROCM_OK) return s;
+    return nrocm_getrs_lookback(h, h->stream, n, nrhs, lu, lda, ipiv, b, ldb, info, mode == NROCM_EXACT, ptr);
 }
