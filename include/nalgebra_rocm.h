@@ -149,11 +149,16 @@ int nrocm_batch_set_graph(nrocm_batch batch, int enable);
  * NULL) receives the chosen lanes per padded size 8, 16, ..., 64 (0 where
  * the batch has no member of that size class). */
 int nrocm_batch_autotune(nrocm_handle handle, nrocm_batch batch, int mode, int *lanes);
+/* Schedule value of the element-tile kernel: one wavefront per matrix with the
+ * lower triangle spread over an 8 x 8 grid of lanes (padded sizes 40..64 and
+ * at most 8 right-hand sides); the autotuner times it beside 8..64 lanes. */
+#define NROCM_SCHED_TILE 100
 /* Current lanes-per-matrix schedule of `mode` in the same format. */
 int nrocm_batch_schedule(nrocm_batch batch, int mode, int *lanes);
 /* Sets the schedule of `mode` (same format; 0 keeps a size class's current
- * choice).  Each entry must be 8, 16, 32 or 64 and at least the padded size;
- * otherwise -3 and nothing changes. */
+ * choice).  Each entry must be 8, 16, 32 or 64 and at least the padded size,
+ * or NROCM_SCHED_TILE where that kernel applies; otherwise -3 and nothing
+ * changes. */
 int nrocm_batch_set_schedule(nrocm_batch batch, int mode, const int *lanes);
 
 /* For each j: Cholesky::new(A_j) then solve_mut(B_j), in one persistent
@@ -162,6 +167,18 @@ int nrocm_batch_set_schedule(nrocm_batch batch, int mode, const int *lanes);
  * info[j] != 0, b[j] is left unchanged. */
 int nrocm_dpotrf_potrs_batched(nrocm_handle handle, nrocm_batch batch, double *const *a,
                                double *const *b, int *info, int mode);
+
+/* For each j: LU::new(A_j) (src/linalg/lu.rs:93-122), then, when the batch has
+ * right-hand sides, LU::solve_mut(B_j) (lu.rs:242-260), one launch per size
+ * class.  a[j] -> n_j x n_j (lda n_j), overwritten by the packed L \ U exactly
+ * as LU::new leaves it; ipiv (NULL or a device array of per-member device
+ * pointers, n_j ints each) receives the 0-based pivot row of every step (i
+ * when nothing was swapped); b[j] -> n_j x nrhs, overwritten by the solution.
+ * info[j] = 0, or 1 + the largest i with U_ii == 0 (solve_mut returns false;
+ * b[j] is then left unchanged).  n_j <= 64.  No reference counterpart as a
+ * batch: the reference runs a loop of LU::new + solve_mut. */
+int nrocm_dgetrf_getrs_batched(nrocm_handle handle, nrocm_batch batch, double *const *a,
+                               int *const *ipiv, double *const *b, int *info, int mode);
 
 /* Adjoint of X_j = A_j^{-1} B_j given the factor L_j, the solution X_j and the
  * upstream gradient dX_j (n_j x nrhs):  Lambda = A^{-1} dX,

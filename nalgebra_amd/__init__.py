@@ -23,10 +23,10 @@ from .linalg import (
     solve_mixed_ir,
     last_ir_fallback,
 )
-from .batched import CholeskyBatch, LeastSquaresBatch
+from .batched import CholeskyBatch, LeastSquaresBatch, LUBatch
 
 __all__ = [
     "EXACT", "FAST", "PREC_FP64", "PREC_FP32", "PREC_BF16", "PREC_BF16X3", "PREC_GMRES", "PREC_WINDOW", "NrocmError", "header_functions",
     "DMatrix", "LU", "Cholesky", "QR", "PermutationSequence", "gemm", "handle", "solve_mixed_ir", "last_ir_fallback",
-    "CholeskyBatch", "LeastSquaresBatch",
+    "CholeskyBatch", "LeastSquaresBatch", "LUBatch",
 ]

@@ -1,5 +1,5 @@
-"""Batched uneven-size solvers: Cholesky + solve with its adjoint, and tall
-least squares.  Members are packed back to back in one device buffer each
+"""Batched uneven-size solvers: Cholesky + solve with its adjoint, LU + solve,
+and tall least squares.  Members are packed back to back in one device buffer each
 (a Vec<DMatrix> flattened), and the C ABI receives device arrays of
 per-member pointers.
 """
@@ -121,6 +121,58 @@ class CholeskyBatch:
         st = lib().nrocm_dpotrs_backward_batched(h, self._batch, _ptr(pL), _ptr(pX), _ptr(pdX),
                                                  _ptr(pdA), _ptr(pdB))
         check("nrocm_dpotrs_backward_batched", st, h)
+
+
+class LUBatch:
+    """count general systems A_j (n_j x n_j, n_j <= 64) with B_j (n_j x nrhs):
+    LU::new + LU::solve_mut per member (nrocm_dgetrf_getrs_batched); pivots
+    packed back to back (member j's n_j pivots at offP[j])."""
+
+    def __init__(self, ns: Sequence[int], nrhs: int, device="cuda"):
+        self.ns = np.ascontiguousarray(ns, dtype=np.int32)
+        self.count = len(self.ns)
+        self.nrhs = int(nrhs)
+        self.device = torch.device(device)
+        n64 = self.ns.astype(np.int64)
+        self.offA = np.concatenate([[0], np.cumsum(n64 * n64)[:-1]]).astype(np.int64)
+        self.offB = np.concatenate([[0], np.cumsum(n64 * self.nrhs)[:-1]]).astype(np.int64)
+        self.offP = np.concatenate([[0], np.cumsum(n64)[:-1]]).astype(np.int64)
+        self.sizeA = int((n64 * n64).sum())
+        self.sizeB = int((n64 * self.nrhs).sum())
+        self.sizeP = int(n64.sum())
+        self._h = handle()
+        bp = C.c_void_p()
+        check("nrocm_batch_create",
+              lib().nrocm_batch_create(self._h, self.count,
+                                       self.ns.ctypes.data_as(C.POINTER(C.c_int)), self.nrhs,
+                                       C.byref(bp)), self._h)
+        self._batch = bp.value
+
+    def __del__(self):
+        b = getattr(self, "_batch", None)
+        if b:
+            try:
+                lib().nrocm_batch_destroy(b)
+            except Exception:
+                pass
+
+    def pointers(self, A: torch.Tensor, B: torch.Tensor, P: torch.Tensor):
+        return _pointer_array(A, self.offA), _pointer_array(B, self.offB), _pointer_array(P, self.offP)
+
+    def solve(self, A: torch.Tensor, B: torch.Tensor, mode: int = EXACT):
+        """Factor every A_j in place (packed L \\ U) and overwrite B_j with
+        A_j^{-1} B_j.  Returns (pivots packed int32, info int32)."""
+        P = torch.empty(max(self.sizeP, 1), dtype=torch.int32, device=self.device)
+        info = torch.empty(self.count, dtype=torch.int32, device=self.device)
+        pa, pb, pp = self.pointers(A, B, P)
+        self.solve_ptrs(pa, pp, pb, info, mode)
+        return P[:self.sizeP], info
+
+    def solve_ptrs(self, pa, pp, pb, info, mode: int = EXACT):
+        h = handle()
+        st = lib().nrocm_dgetrf_getrs_batched(h, self._batch, _ptr(pa), _ptr(pp) if pp is not None else None,
+                                              _ptr(pb), _ptr(info), mode)
+        check("nrocm_dgetrf_getrs_batched", st, h)
 
 
 class LeastSquaresBatch:

@@ -25,7 +25,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["chol_batched.hip", *[f"chol_nb{nb:02d}.hip" for nb in range(8, 65, 8)],
-               "lu.hip", "gemm.hip", "qr.hip", "mixed.hip", "solve_lookback.hip"]
+               "lu.hip", "gemm.hip", "qr.hip", "mixed.hip", "solve_lookback.hip", "lu_batched.hip"]
 HOST_SOURCES = ["runtime.cpp"]
 
 HIP_FLAGS = [

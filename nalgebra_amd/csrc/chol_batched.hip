@@ -15,6 +15,7 @@
 // Identity padding does not change the leading n x n results: padded rows
 // and columns only ever meet zeros, and the dot lengths use the true n.
 #include "chol_kernels.hpp"
+#include "chol_tile.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -52,6 +53,22 @@ const FusedLaunch kFused[8] = {nrocm_chol_fused_launch_8,  nrocm_chol_fused_laun
                                nrocm_chol_fused_launch_40, nrocm_chol_fused_launch_48,
                                nrocm_chol_fused_launch_56, nrocm_chol_fused_launch_64};
 
+using TileLaunch = int (*)(bool, hipStream_t, const int *, int, int, const int *, double *const *, double *const *,
+                          int, int *);
+// the element-tile schedule of padded sizes 40..64 (bins 4..7)
+const TileLaunch kTile[4] = {nrocm_chol_tile_launch_40, nrocm_chol_tile_launch_48, nrocm_chol_tile_launch_56,
+                             nrocm_chol_tile_launch_64};
+
+bool tile_ok(int bin, int nrhs) { return bin >= 4 && bin < 8 && nrhs <= nrocm_chol_tile::MAXRHS; }
+
+// One fused launch of a bin's members [begin, end) of `order` with schedule G.
+int launch_bin(bool exact, int G, int bin, hipStream_t st, const int *order, int begin, int end, const int *nsz,
+               double *const *A, double *const *B, int nrhs, int *info)
+{
+    if (G == NROCM_SCHED_TILE) return kTile[bin - 4](exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    return kFused[bin](exact, G, st, order, begin, end, nsz, A, B, nrhs, info);
+}
+
 // Default lanes per matrix: the smallest power-of-two group holding the rows.
 int default_lanes(int bin)
 {
@@ -70,8 +87,8 @@ int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const
         const int mpw = mpw_of_bin(bin.kernel);
         (void)mpw;
         const int G = bt->lanes[exact ? 0 : 1][&bin - bt->bins.data()];
-        if (kFused[bin.kernel](exact, G, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
-                               bt->nrhs > 0 ? B : nullptr, bt->nrhs, info))
+        if (launch_bin(exact, G, bin.kernel, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
+                       bt->nrhs > 0 ? B : nullptr, bt->nrhs, info))
             NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;
@@ -123,9 +140,12 @@ int autotune(nrocm_handle h, nrocm_batch bt, bool exact)
     int status = NROCM_OK;
     for (size_t bi = 0; bi < bt->bins.size() && status == NROCM_OK; ++bi) {
         const auto &bin = bt->bins[bi];
-        const int cand[4] = {8, 16, 32, 64};
+        const int cand[5] = {8, 16, 32, 64, NROCM_SCHED_TILE};
+        auto usable = [&](int G) {
+            return G == NROCM_SCHED_TILE ? tile_ok(bin.kernel, bt->nrhs) : G >= bin.nb;
+        };
         int ncand = 0;
-        for (int G : cand) ncand += (G >= bin.nb) ? 1 : 0;
+        for (int G : cand) ncand += usable(G) ? 1 : 0;
         if (ncand < 2) continue;
         const int k = std::min(bin.end - bin.begin, kTuneSample);
         std::vector<int> ns(k);
@@ -169,13 +189,13 @@ int autotune(nrocm_handle h, nrocm_batch bt, bool exact)
         float best = 0.0f;
         int bestG = bt->lanes[mi][bi];
         for (int G : cand) {
-            if (G < bin.nb || e != hipSuccess) continue;
+            if (!usable(G) || e != hipSuccess) continue;
             float tmin = 1e30f;
             for (int rep = 0; rep <= kReps && e == hipSuccess; ++rep) {  // rep 0 warms up
                 hipLaunchKernelGGL(tune_fill_kernel, dim3(k), dim3(256), 0, st, dn, pA, Bp, bt->nrhs);
                 e = hipGetLastError();
                 if (e == hipSuccess) e = hipEventRecord(e0, st);
-                if (e == hipSuccess && kFused[bin.kernel](exact, G, st, dord, 0, k, dn, pA, Bp, bt->nrhs, dinfo))
+                if (e == hipSuccess && launch_bin(exact, G, bin.kernel, st, dord, 0, k, dn, pA, Bp, bt->nrhs, dinfo))
                     e = hipErrorLaunchFailure;
                 if (e == hipSuccess) e = hipEventRecord(e1, st);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -331,7 +351,11 @@ int nrocm_batch_set_schedule(nrocm_batch bt, int mode, const int *lanes)
         return nrocm_fail(bt->handle, NROCM_ERR_UNSUPPORTED, "batched Cholesky supports n <= 64");
     for (int c = 0; c < 8; ++c) {
         const int G = lanes[c];
-        if (G != 0 && ((G != 8 && G != 16 && G != 32 && G != 64) || G < 8 * (c + 1))) return -3;
+        if (G == NROCM_SCHED_TILE) {
+            if (!tile_ok(c, bt->nrhs)) return -3;
+        } else if (G != 0 && ((G != 8 && G != 16 && G != 32 && G != 64) || G < 8 * (c + 1))) {
+            return -3;
+        }
     }
     const int mi = mode == NROCM_EXACT ? 0 : 1;
     for (size_t bi = 0; bi < bt->bins.size(); ++bi)

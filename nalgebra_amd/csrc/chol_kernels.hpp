@@ -15,13 +15,14 @@ constexpr int group_for() { return NB <= 8 ? 8 : NB <= 16 ? 16 : NB <= 32 ? 32 :
 template <bool EXACT>
 __device__ __forceinline__ double sub_prod(double acc, double x, double y)
 {
-    // nalgebra axpy with factor -x: fl(fl(-x*y) + acc) == fl(acc - fl(x*y)).
-    // One asm block per update keeps the product next to its subtraction: as
+    // nalgebra axpy with factor -x: fl(fl((-x)*y) + acc), the negation on x
+    // before the product as there (a NaN coming from y keeps its sign).
+    // One asm block per update keeps the product next to its addition: as
     // separate IR operations the scheduler hoists every product of a step and
     // spills the row array to scratch.
     double r;
     if constexpr (EXACT) {
-        asm("v_mul_f64 %0, %1, %2\n\tv_add_f64 %0, %3, -%0" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+        asm("v_mul_f64 %0, -%1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
     } else {
         // as an asm statement for the same reason: a plain fma lets the
         // scheduler raise register use to 256 VGPRs

@@ -1,0 +1,242 @@
+// Batched (uneven sizes, n <= 64) LU with partial pivoting and solve:
+// LU::new (src/linalg/lu.rs:93-122) then LU::solve_mut (lu.rs:242-260) per
+// member, one launch per size class of the batch (padded sizes 8, 16, ..., 64).
+//
+// Layout: a group of G lanes serves one matrix (64 / G matrices per
+// wavefront); lane r holds row r of its matrix in registers for the whole
+// kernel.  Rows never move: each lane carries the current position of its
+// row, and a swap of positions i and piv exchanges the two lanes' positions,
+// which is nalgebra's full-row swap (lu.rs:114 swaps the left part too,
+// gauss_step_swap :362-389 the rest).  Step i:
+//   1. pivot: the first position >= i holding the strictly largest |a_i|
+//      (icamax, src/base/min_max.rs:221-241; a NaN at position i wins, other
+//      NaNs never do), reduced over the group;
+//   2. a zero pivot skips the step (lu.rs:107-110);
+//   3. the pivot row is broadcast through LDS; rows below take
+//      l = fl(a_i * fl(1 / d)) and a_k = fl(a_k - fl(u_k * l)) for k > i, the
+//      same unfused per-element sequence as gauss_step (NROCM_EXACT), or one
+//      fused multiply-add (NROCM_FAST).
+// The solve keeps the rows where they are: the right-hand side value of the
+// row at position p is b_r of its original row r (PermutationSequence::
+// permute_rows applies the same swaps), forward substitution with the unit
+// lower factor (solve.rs:106-133) and backward substitution with U
+// (solve.rs:137-182) broadcast the solved value of position i from the lane
+// holding that position, and x is stored at the row's final position.
+#include "internal_launch.hpp"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int CH = 4;  // right-hand sides held in registers at a time
+
+template <int NB>
+constexpr int group_for() { return NB <= 8 ? 8 : NB <= 16 ? 16 : NB <= 32 ? 32 : 64; }
+
+template <bool EXACT>
+__device__ __forceinline__ double sub_prod(double acc, double x, double y)
+{
+    // nalgebra's axpy with factor -x: fl(fl((-x) * y) + acc).  The negation
+    // is applied to x before the product, as there, so that a NaN coming from
+    // y keeps its sign.  One asm block keeps the product next to its addition
+    // (as two IR operations the scheduler hoists every product of a step and
+    // spills the row array).
+    double r;
+    if constexpr (EXACT) {
+        asm("v_mul_f64 %0, -%1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+    } else {
+        asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(acc));
+    }
+    return r;
+}
+
+// (value, position) under icamax's rule: larger value, then lower position.
+__device__ __forceinline__ bool beats(double av, int ap, double bv, int bp)
+{
+    return av > bv || (av == bv && ap < bp);
+}
+
+// Group-wide pivot: every lane of the group ends with the winner.
+template <int G>
+__device__ __forceinline__ void group_argmax(double &v, int &pos)
+{
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(v, off, G);
+        const int op = __shfl_xor(pos, off, G);
+        if (beats(ov, op, v, pos)) {
+            v = ov;
+            pos = op;
+        }
+    }
+}
+
+__device__ const double kZeroCellLU[1] = {0.0};
+
+template <int NB, int G, bool EXACT>
+__global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
+    const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz, double *const *__restrict__ Aptr,
+    int *const *__restrict__ Pptr, double *const *__restrict__ Bptr, int nrhs, int *__restrict__ info)
+{
+    constexpr int MPW = 64 / G;
+    __shared__ double rowbuf[MPW][NB];
+    __shared__ double bbuf[MPW][CH];
+    __shared__ int where[MPW][NB];  // final position -> lane within the group
+    const int lane = threadIdx.x;
+    const int g = lane / G, r = lane % G;
+    const int item = begin + (int)blockIdx.x * MPW + g;
+    const bool valid = item < end;
+    const int jm = valid ? order[item] : 0;
+    const int n = valid ? nsz[jm] : 0;
+    double *const A = valid ? Aptr[jm] : nullptr;
+    int *const P = (valid && Pptr) ? Pptr[jm] : nullptr;
+    const bool row_ok = r < n;
+
+    double a[NB];
+    {
+        const double *row = row_ok ? A + r : kZeroCellLU;
+        const size_t step = row_ok ? (size_t)n : 0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const double *src = (c < n) ? row + (size_t)c * step : kZeroCellLU;
+            a[c] = *src;
+        }
+    }
+    int pos = r;
+    int zmax = 0;  // 1 + the largest step with a zero pivot (U_ii == 0)
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const bool live = i < n;  // this group still has step i
+        // 1. pivot search over the positions >= i
+        double v = __builtin_fabs(a[i]);
+        v = (v != v) ? ((pos == i) ? __builtin_inf() : -1.0) : v;
+        const bool cand = live && row_ok && pos >= i;
+        v = cand ? v : -1.0;
+        int bp = cand ? pos : 0x7fffffff;
+        group_argmax<G>(v, bp);
+        const bool is_piv = cand && pos == bp;
+        // 3. the pivot row (its entries from column i on) through LDS
+        if (is_piv) {
+#pragma unroll
+            for (int k = i; k < NB; ++k) rowbuf[g][k] = a[k];
+        }
+        __syncthreads();
+        const double d = live ? rowbuf[g][i] : 1.0;
+        if (live && r == 0 && P) P[i] = (d == 0.0) ? i : bp;
+        if (live && d == 0.0) zmax = i + 1;  // 2. zero pivot: nothing moves
+        if (live && d != 0.0) {
+            if (pos == i) pos = bp;
+            else if (is_piv) pos = i;
+            if (pos > i && row_ok) {
+                const double inv = 1.0 / d;
+                const double l = a[i] * inv;
+                a[i] = l;
+#pragma unroll
+                for (int k = i + 1; k < NB; ++k) a[k] = sub_prod<EXACT>(a[k], rowbuf[g][k], l);
+            }
+        }
+        __syncthreads();
+    }
+    // the factors, each row at its final position
+    if (row_ok) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < n) A[pos + (size_t)c * n] = a[c];
+    }
+    if (valid && r == 0) info[jm] = zmax;
+    if (nrhs <= 0 || Bptr == nullptr) return;
+
+    // ---- solve: skipped for a singular U (solve_mut returns false), whose
+    // right-hand sides are left unchanged
+    double *const B = valid ? Bptr[jm] : nullptr;
+    const bool solve = valid && zmax == 0;
+    if (row_ok) where[g][pos] = r;
+    __syncthreads();
+    for (int c0 = 0; c0 < nrhs; c0 += CH) {
+        double b[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) b[q] = (solve && row_ok && c0 + q < nrhs) ? B[r + (size_t)(c0 + q) * n] : 0.0;
+        // forward: unit lower, per element the steps i = 0, 1, ... in order
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const bool live = i + 1 < n;
+            if (live && where[g][i] == r) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) bbuf[g][q] = b[q];
+            }
+            __syncthreads();
+            if (live && pos > i && row_ok) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
+            }
+            __syncthreads();
+        }
+        // backward: x_i = b_i / U_ii, then the positions above it
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            const bool live = i < n;
+            if (live && where[g][i] == r) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    b[q] = b[q] / a[i];
+                    bbuf[g][q] = b[q];
+                }
+            }
+            __syncthreads();
+            if (live && pos < i && row_ok) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
+            }
+            __syncthreads();
+        }
+        if (solve && row_ok) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+                if (c0 + q < nrhs) B[pos + (size_t)(c0 + q) * n] = b[q];
+        }
+    }
+}
+
+template <int NB>
+int launch_nb(bool exact, hipStream_t st, const int *order, int begin, int end, const int *nsz, double *const *A,
+              int *const *P, double *const *B, int nrhs, int *info)
+{
+    constexpr int G = group_for<NB>();
+    constexpr int MPW = 64 / G;
+    const unsigned grid = (unsigned)((end - begin + MPW - 1) / MPW);
+    if (grid == 0) return 0;
+    if (exact)
+        hipLaunchKernelGGL((getrf_getrs_batched_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st, order, begin, end,
+                           nsz, A, P, B, nrhs, info);
+    else
+        hipLaunchKernelGGL((getrf_getrs_batched_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st, order, begin,
+                           end, nsz, A, P, B, nrhs, info);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+using Launch = int (*)(bool, hipStream_t, const int *, int, int, const int *, double *const *, int *const *,
+                       double *const *, int, int *);
+const Launch kLaunch[8] = {launch_nb<8>, launch_nb<16>, launch_nb<24>, launch_nb<32>,
+                           launch_nb<40>, launch_nb<48>, launch_nb<56>, launch_nb<64>};
+
+}  // namespace
+
+extern "C" int nrocm_dgetrf_getrs_batched(nrocm_handle h, nrocm_batch bt, double *const *a, int *const *ipiv,
+                                          double *const *b, int *info, int mode)
+{
+    if (!h) return -1;
+    if (!bt || bt->handle != h) return -2;
+    if (bt->count > 0 && !a) return -3;
+    if (bt->count > 0 && bt->nrhs > 0 && !b) return -5;
+    if (bt->count > 0 && !info) return -6;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -7;
+    if (bt->count == 0) return NROCM_OK;
+    if (bt->max_n > 64) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "batched LU supports n <= 64");
+    for (const auto &bin : bt->bins) {
+        if (bin.end <= bin.begin) continue;
+        if (kLaunch[bin.kernel](mode == NROCM_EXACT, h->stream, bt->order_dev, bin.begin, bin.end, bt->n_dev, a, ipiv,
+                                bt->nrhs > 0 ? b : nullptr, bt->nrhs, info))
+            NROCM_CHECK_LAUNCH(h);
+    }
+    return NROCM_OK;
+}

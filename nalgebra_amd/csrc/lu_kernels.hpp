@@ -95,12 +95,23 @@ __device__ __forceinline__ T pass(T v)
     return v;
 }
 
+// nalgebra's axpy with factor -x: fl(fl((-x) * y) + acc), the negation on x
+// before the product as in gauss_step (lu.rs:353, :386).  In FP64 the operand
+// order is pinned (-x first in the product, the product first in the sum), as
+// the reference's x86 code has it, so that NaN results carry the same sign
+// (with two NaN operands the first one is returned); NROCM_FAST fuses it.
 template <typename T>
 __device__ __forceinline__ T sub_prod(T acc, T x, T y, bool exact)
 {
     if (exact) {
-        T p = x * y;
-        return acc - p;
+        if constexpr (sizeof(T) == 8) {
+            double r;
+            asm("v_mul_f64 %0, -%1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+            return r;
+        } else {
+            const T p = (-x) * y;
+            return p + acc;
+        }
     }
     return fma(-x, y, acc);
 }

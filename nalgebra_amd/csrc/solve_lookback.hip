@@ -32,10 +32,12 @@ constexpr int PERM_LDS_MAX = 16384;       // composed permutation kept in LDS up
 template <bool EXACT>
 __device__ __forceinline__ double sub_prod(double acc, double x, double y)
 {
-    // fl(fl(-x * y) + acc) == fl(acc - fl(x * y)); FAST fuses it
+    // nalgebra's axpy with factor -x: fl(fl((-x) * y) + acc) (the negation on
+    // x before the product, so a NaN coming from y keeps its sign); FAST fuses it
     if constexpr (EXACT) {
-        const double p = x * y;
-        return acc - p;
+        double r;  // operand order pinned: NaN results carry the reference's sign
+        asm("v_mul_f64 %0, -%1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+        return r;
     } else {
         return fma(-x, y, acc);
     }

@@ -24,10 +24,12 @@ def _rel(a, b):
 NS_ALL = list(range(1, 65)) + [64] * 9 + [33] * 5 + [4] * 11 + list(range(64, 0, -7))
 
 
-def _run_batch(nl, ns, nrhs, seed, mode, graph=True):
+def _run_batch(nl, ns, nrhs, seed, mode, graph=True, schedule=None):
     A, offA, B, offB = gen.spd_batch(seed, ns, nrhs)
     bt = nl.CholeskyBatch(ns, nrhs)
     bt.set_graph(graph)
+    if schedule:
+        bt.set_schedule(schedule, mode)
     Ad = torch.from_numpy(A).cuda()
     Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, mode)
@@ -35,11 +37,14 @@ def _run_batch(nl, ns, nrhs, seed, mode, graph=True):
     return A, offA, B, offB, Ad.cpu().numpy(), Bd.cpu().numpy(), info.cpu().numpy()
 
 
-@pytest.mark.parametrize("nrhs", [4, 1, 6])
-def test_batched_exact_is_bitwise(nl, orc, nrhs):
+@pytest.mark.parametrize("nrhs", [4, 1, 6, 0, 3])
+@pytest.mark.parametrize("sched", ["default", "rows", "tile"])
+def test_batched_exact_is_bitwise(nl, orc, nrhs, sched):
     ns = NS_ALL
-    print(f"batched exact: {len(ns)} members, nrhs={nrhs}", flush=True)
-    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 11, nl.EXACT)
+    print(f"batched exact: {len(ns)} members, nrhs={nrhs}, schedule {sched}", flush=True)
+    sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
+           "tile": {nb: TILE for nb in range(40, 65, 8)} if nrhs <= 4 else None}[sched]
+    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 11, nl.EXACT, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
     assert np.array_equal(info, info_r) and not info.any()
@@ -47,15 +52,21 @@ def test_batched_exact_is_bitwise(nl, orc, nrhs):
     assert np.array_equal(_bits(Bg), _bits(Br)), "solution bits differ"
 
 
-@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+TILE = 100  # NROCM_SCHED_TILE: one wavefront per matrix, 8 x 8 lane grid (padded sizes 40..64)
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64, TILE])
 def test_every_schedule_is_bitwise(nl, orc, lanes):
-    """Each compiled lanes-per-matrix schedule (the autotuner may pick any)
-    reproduces the oracle bit for bit; size classes that do not fit `lanes`
-    keep their default."""
+    """Each compiled schedule (the autotuner may pick any) reproduces the
+    oracle bit for bit; size classes that do not fit `lanes` keep their
+    default."""
     ns = NS_ALL
     A, offA, B, offB = gen.spd_batch(21, ns, 4)
     bt = nl.CholeskyBatch(ns, 4)
-    want = {nb: lanes for nb in range(8, 65, 8) if nb <= lanes}
+    if lanes == TILE:
+        want = {nb: TILE for nb in range(40, 65, 8)}
+    else:
+        want = {nb: lanes for nb in range(8, 65, 8) if nb <= lanes}
     bt.set_schedule(want, nl.EXACT)
     got = bt.schedule(nl.EXACT)
     for nb, g in got.items():
@@ -77,7 +88,7 @@ def test_autotune_keeps_parity(nl, orc):
     print("autotuned lanes per matrix:", chosen, flush=True)
     assert set(chosen) == {8 * ((n + 7) // 8) for n in ns}
     for nb, g in chosen.items():
-        assert g in (8, 16, 32, 64) and g >= nb
+        assert (g in (8, 16, 32, 64) and g >= nb) or (g == TILE and nb >= 40)
     assert bt.schedule(nl.EXACT) == chosen
     Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
@@ -90,9 +101,12 @@ def test_autotune_keeps_parity(nl, orc):
         bt.set_schedule({64: 32}, nl.EXACT)
 
 
-def test_batched_fast_within_1e12(nl, orc):
+@pytest.mark.parametrize("sched", ["default", "rows", "tile"])
+def test_batched_fast_within_1e12(nl, orc, sched):
     ns = NS_ALL
-    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST)
+    sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
+           "tile": {nb: TILE for nb in range(40, 65, 8)}}[sched]
+    A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
     assert not info.any()
@@ -122,23 +136,32 @@ def test_batched_empty(nl):
     assert info.numel() == 0
 
 
-def test_batched_not_positive_definite(nl, orc):
-    ns = [5, 17, 40, 64, 3]
+@pytest.mark.parametrize("sched", ["rows", "tile"])
+def test_batched_not_positive_definite(nl, orc, sched):
+    ns = [5, 17, 40, 64, 3, 57, 48]
     A, offA, B, offB = gen.spd_batch(13, ns, 4)
-    # member 2: make column 7's pivot negative after elimination; member 4: NaN diagonal
+    # member 2: make column 7's pivot negative after elimination; member 4: NaN
+    # diagonal; member 5: NaN in column 30 below the diagonal (pivot 31 turns NaN);
+    # member 6: an exactly zero pivot at column 47
     M = A[offA[2]:offA[2] + 1600].reshape(40, 40, order="F")
     M[7, 7] = -100.0
     A[offA[4]] = np.nan
+    A[offA[5] + 40 + 30 * 57] = np.nan
+    M6 = A[offA[6]:offA[6] + 48 * 48].reshape(48, 48, order="F")
+    L6 = np.linalg.cholesky(M6)
+    M6[47, 47] -= L6[47, 47] ** 2  # the last pivot becomes (about) zero or negative
     bt = nl.CholeskyBatch(ns, 4)
+    bt.set_schedule({nb: (64 if sched == "rows" else TILE) for nb in range(40, 65, 8)}, nl.EXACT)
     Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
     Ar, Br = A.copy(), B.copy()
     info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
-    assert list(info) == list(info_r) == [0, 0, 8, 0, 1]
+    assert list(info) == list(info_r)
+    assert list(info_r[:5]) == [0, 0, 8, 0, 1] and info_r[5] == 41 and info_r[6] in (0, 48)
     Bg = Bd.cpu().numpy()
     # a failed member's right-hand side is left unchanged; others are solved exactly
     assert np.array_equal(Bg[offB[2]:offB[2] + 160], B[offB[2]:offB[2] + 160])
-    for j in (0, 1, 3):
+    for j in [j for j in range(len(ns)) if info_r[j] == 0]:
         sl = slice(offB[j], offB[j] + 4 * ns[j])
         assert np.array_equal(_bits(Bg[sl]), _bits(Br[sl]))
 

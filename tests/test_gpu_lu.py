@@ -95,6 +95,31 @@ def test_lu_exact_ties_and_shapes(nl, orc, kind, shape):
         assert np.array_equal(_bits(x.numpy()), _bits(xr)), "solution bits differ"
 
 
+@pytest.mark.parametrize("shape,where", [((40, 40), "pivot"), ((40, 40), "below"), ((90, 90), "row"),
+                                         ((200, 200), "below"), ((300, 300), "pivot")])
+def test_lu_exact_nan_bits(nl, orc, shape, where):
+    """NaN inputs: LU::new never fails, so its NaN factors are observable.  A
+    NaN at the pivot position wins the search (min_max.rs:227-240), other
+    NaNs never do: pivots, NaN positions and every other bit match the
+    oracle.  The sign of a NaN produced from two NaN operands is left open
+    (x86 returns the first operand, the GPU does not), so NaNs compare as NaNs."""
+    m, n = shape
+    A = gen.uniform(900 + m, shape)
+    if where == "pivot":
+        A[0, 0] = np.nan
+    elif where == "below":
+        A[m // 2, 3] = np.nan
+    else:
+        A[5, :] = np.nan
+    lu = nl.DMatrix.from_numpy(A).lu(nl.EXACT)
+    LUr, ipr, infor = orc.getrf(A)
+    assert np.array_equal(lu.ipiv[:min(m, n)].cpu().numpy(), ipr)
+    G = lu.lu.numpy()
+    assert np.array_equal(np.isnan(G), np.isnan(LUr)), "NaN positions differ"
+    ok = ~np.isnan(LUr)
+    assert np.isnan(LUr).any() and np.array_equal(_bits(G[ok]), _bits(LUr[ok])), "LU bits differ"
+
+
 def test_config1_gemm_lu_solve(nl, orc):
     """f64 512x512, entries U[-1,1]: C = A*B, LU of A, solve A x = b."""
     n = 512
