@@ -32,12 +32,10 @@
 namespace nrocm_chol_tile {
 
 using nrocm_chol::div_den_ok;
-using nrocm_chol::div_num_ok;
 using nrocm_chol::div_with;
 using nrocm_chol::lane_value;
 using nrocm_chol::recip_refined;
 using nrocm_chol::row_shl;
-using nrocm_chol::sub_prod;
 
 constexpr int MAXRHS = 4;  // right-hand sides carried through the factor (one per lane row)
 
@@ -55,6 +53,36 @@ struct TileSmem {
 
 __device__ const double kZeroCellT[2] = {0.0, 1.0};  // a zero, then a one for padded diagonals
 
+// Global-address-space views of the member pointers (which come from a pointer
+// array, so the compiler would otherwise emit flat instructions, and flat
+// instructions also count against the LDS counter that every step's LDS
+// barrier drains: each step would wait for its own column store).
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double gcdouble;
+
+// The update a = fl(fl((-x) * y) + a) in place (nalgebra's axpy with factor
+// -x), or one fused multiply-add in FAST.  Updating the accumulator register
+// itself keeps exec-masked updates free of result copies.
+template <bool EXACT>
+__device__ __forceinline__ void upd(double &acc, double x, double y)
+{
+    if constexpr (EXACT) {
+        double tmp;
+        asm("v_mul_f64 %1, -%2, %3\n\tv_add_f64 %0, %1, %0" : "+v"(acc), "=&v"(tmp) : "v"(x), "v"(y));
+    } else {
+        asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(acc) : "v"(x), "v"(y));
+    }
+}
+
+// Numerator range of div_with from the exponent field alone: 2^-900 <= |x| <
+// 2^600, or +0 (for which the three-instruction form also gives +0).
+__device__ __forceinline__ bool num_in_range(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned e = (unsigned)(b >> 52) & 0x7ffu;
+    return (e - 123u) <= 1499u || b == 0ull;
+}
+
 // Exact quotient v / den for a wavefront-uniform den with reciprocal y: the
 // three-instruction form when every lane's operands are in its range, the
 // compiler's division otherwise (see div_with).
@@ -62,7 +90,7 @@ template <bool EXACT>
 __device__ __forceinline__ double quotient(double v, double den, double y, bool den_ok)
 {
     if constexpr (EXACT) {
-        const bool ok = den_ok && div_num_ok(v);
+        const bool ok = den_ok && num_in_range(v);
         if (__ballot(!ok) == 0) return div_with(v, den, y);
         return v / den;
     } else {
@@ -70,8 +98,13 @@ __device__ __forceinline__ double quotient(double v, double den, double y, bool 
     }
 }
 
+// Wavefronts per SIMD of each padded size (the tiles shrink with NB); at
+// NB = 64 the fourth wavefront costs 3 spilled vector registers.
+template <int NB>
+constexpr int tile_waves() { return NB >= 56 ? 4 : NB == 48 ? 5 : 6; }
+
 template <int NB, bool EXACT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tile_waves<NB>())))
 void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
                              double *const *__restrict__ Aptr, double *const *__restrict__ Bptr, int nrhs,
                              int *__restrict__ info)
@@ -84,10 +117,11 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
     if (item >= end) return;  // one matrix per workgroup: the whole workgroup leaves
     const int jm = order[item];
     const int n = nsz[jm];
-    double *const A = Aptr[jm];
+    gdouble *const A = (gdouble *)Aptr[jm];
     const int lda = n;
     const int nr = (Bptr != nullptr) ? nrhs : 0;
-    double *const B = nr > 0 ? Bptr[jm] : nullptr;
+    gdouble *const B = nr > 0 ? (gdouble *)Bptr[jm] : nullptr;
+    gcdouble *const zc = (gcdouble *)kZeroCellT;
     const int ldb = n;
 
     // ---- load: tiles of the lower triangle (identity padding), right-hand-side rows
@@ -99,22 +133,25 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
             const int r = p + 8 * ri, c = q + 8 * ci;
             const bool in = r < n && c < n && r >= c;
             // padding rows and columns hold the identity
-            const double *src = in ? A + r + (size_t)c * lda : kZeroCellT + (r == c ? 1 : 0);
+            gcdouble *src = in ? A + r + (size_t)c * lda : zc + (r == c ? 1 : 0);
             const double v = *src;
             T[ri][ci] = v;
         }
+        // a tile row of loads at a time: all 36 addresses at once would need
+        // more registers than the factor itself
+        __builtin_amdgcn_sched_barrier(0);
     }
     double Y[R];
 #pragma unroll
     for (int ci = 0; ci < R; ++ci) {
         const int c = q + 8 * ci;
         const bool in = p < nr && c < n;
-        const double *src = in ? B + c + (size_t)p * ldb : kZeroCellT;
+        gcdouble *src = in ? B + c + (size_t)p * ldb : zc;
         Y[ci] = *src;
     }
 
     // ---- factor + forward substitution
-    int fail = 0;
+    unsigned long long bad = 0;  // bit k: pivot k (< n) is not > 0
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const int kq = k & 7, kc = k >> 3;
@@ -126,7 +163,7 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
         const int p = lane & 7, q = lane >> 3;
         // 1. pivot
         const double d = lane_value(T[kc][kc], kq * 9);
-        if (k < n && fail == 0 && !(d > 0.0)) fail = k + 1;
+        bad |= (unsigned long long)(k < n && !(d > 0.0)) << k;
         const double den = sqrt(d);
         const double y = EXACT ? recip_refined(den) : 1.0 / den;
         const bool den_ok = div_den_ok(den);
@@ -142,21 +179,35 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
         }
         lds_barrier();
         double lk;
-        {
-            // rows that are not divided (above the diagonal, padding) use a
-            // benign numerator so that they never force the slow division
-            const bool row = lane > k && lane < n;
+        // rows that are not divided (above the diagonal, padding) use a benign
+        // numerator so that they never force the slow division
+        const bool row = lane > k && lane < n;
+        if (k >= MAXRHS - 1) {
+            // lanes 0..k have no row to divide: lane c < nr divides right-hand
+            // side c's coefficient instead, one quotient per lane for both
+            const bool rhs = lane < nr;
+            const double v = s.colraw[row ? lane : (rhs ? 64 + lane : 0)];
+            const double qv = quotient<EXACT>((row || rhs) ? v : 1.0, den, y, den_ok);
+            lk = row ? qv : (lane == k ? den : 0.0);
+            if (rhs) {
+                s.ycol[lane] = qv;
+                s.ysave[lane][k] = qv;
+            }
+        } else {
             const double v = row ? s.colraw[lane] : 1.0;
             const double qv = quotient<EXACT>(v, den, y, den_ok);
             lk = row ? qv : (lane == k ? den : 0.0);
+            if (lane < nr) {
+                const double yk = quotient<EXACT>(s.colraw[64 + lane], den, y, den_ok);
+                s.ycol[lane] = yk;
+                s.ysave[lane][k] = yk;
+            }
         }
-        if (lane >= k && lane < n && k < n) A[lane + (size_t)k * lda] = lk;
+        if (k < n) {
+            gdouble *const colk = A + (size_t)k * lda;  // uniform base, lane offset
+            if (lane >= k && lane < n) colk[lane] = lk;
+        }
         if (q < R) s.colbuf[p][q] = lk;  // row `lane` = p + 8 q sits at [p][q]
-        if (lane < nr) {
-            const double yk = quotient<EXACT>(s.colraw[64 + lane], den, y, den_ok);
-            s.ycol[lane] = yk;
-            s.ysave[lane][k] = yk;
-        }
         lds_barrier();
         // 3. multipliers and the trailing update
         if (kc < R) {
@@ -171,8 +222,8 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
             for (int ci = kc; ci < R; ++ci) {
                 if (ci > kc || q > kq) {
 #pragma unroll
-                    for (int ri = ci; ri < R; ++ri) T[ri][ci] = sub_prod<EXACT>(T[ri][ci], lc[ci], lr[ri]);
-                    Y[ci] = sub_prod<EXACT>(Y[ci], yv, lc[ci]);
+                    for (int ri = ci; ri < R; ++ri) upd<EXACT>(T[ri][ci], lc[ci], lr[ri]);
+                    upd<EXACT>(Y[ci], yv, lc[ci]);
                 }
                 // one tile column at a time: left free, the scheduler
                 // interleaves whole steps and runs out of registers
@@ -180,6 +231,8 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
             }
         }
     }
+    // nalgebra stops at the first pivot that is not > 0 (cholesky.rs:240-270)
+    const int fail = bad ? (int)__builtin_ctzll(bad) + 1 : 0;
     if (lane == 0) info[jm] = fail;
     if (nr == 0 || fail != 0) return;  // failed members keep their right-hand sides
 
@@ -193,7 +246,7 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
     // the diagonal and past n read a zero cell
     constexpr int PF = 8;
     auto lval = [&](int i) {
-        const double *src = (row_ok && i >= 0 && r >= i) ? A + r + (size_t)i * lda : kZeroCellT;
+        gcdouble *src = (row_ok && i >= 0 && r >= i) ? A + r + (size_t)i * lda : zc;
         return *src;
     };
     double cur[PF], nxt[PF];
@@ -238,7 +291,7 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
                 const double dii = s.dg[i];
                 double xq;
                 if constexpr (EXACT) {
-                    const bool ok = div_den_ok(dii) && div_num_ok(num);
+                    const bool ok = div_den_ok(dii) && num_in_range(num);
                     xq = ok ? div_with(num, dii, s.inv[i]) : num / dii;
                 } else {
                     xq = num * s.inv[i];
