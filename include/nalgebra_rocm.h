@@ -180,6 +180,20 @@ int nrocm_dpotrf_potrs_batched(nrocm_handle handle, nrocm_batch batch, double *c
 int nrocm_dgetrf_getrs_batched(nrocm_handle handle, nrocm_batch batch, double *const *a,
                                int *const *ipiv, double *const *b, int *info, int mode);
 
+/* Adjoint of X_j = A_j^{-1} B_j given LU::new's packed factors lu[j] and
+ * pivots ipiv[j] (as nrocm_dgetrf_getrs_batched leaves them), the solution
+ * X_j and the upstream gradient dX_j (n_j x nrhs):
+ *   Lambda = A^{-T} dX = P^T L^{-T} U^{-T} dX, dB = Lambda, dA = -Lambda X^T
+ * (n_j x n_j; products summed over right-hand sides in order).  The solves
+ * are nalgebra's tr_solve_upper_triangular / tr_solve_lower_triangular
+ * (src/linalg/solve.rs:232-276,379-440, dotx order) and the permutation its
+ * inv_permute_rows (permutation_sequence.rs:108-116), bit for bit.  U must be
+ * nonsingular.  No reference counterpart (nalgebra has no autodiff). */
+int nrocm_dgetrs_backward_batched(nrocm_handle handle, nrocm_batch batch,
+                                  const double *const *lu, const int *const *ipiv,
+                                  const double *const *x, const double *const *dx,
+                                  double *const *da, double *const *db);
+
 /* Adjoint of X_j = A_j^{-1} B_j given the factor L_j, the solution X_j and the
  * upstream gradient dX_j (n_j x nrhs):  Lambda = A^{-1} dX,
  * dB = Lambda, dA = -1/2 (Lambda X^T + X Lambda^T) (symmetrised, full n x n).

@@ -60,6 +60,7 @@ _PROTOS = {
     "nrocm_dpotrf_potrs_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
     "nrocm_dpotrs_backward_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nrocm_dgetrf_getrs_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "nrocm_dgetrs_backward_batched": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nrocm_dgeqrf": (_i, [_vp, _i, _i, _vp, _i, _vp]),
     "nrocm_sgeqrf": (_i, [_vp, _i, _i, _vp, _i, _vp]),
     "nrocm_dqr_qtmul": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i]),

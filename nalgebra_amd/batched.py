@@ -168,6 +168,18 @@ class LUBatch:
         self.solve_ptrs(pa, pp, pb, info, mode)
         return P[:self.sizeP], info
 
+    def backward(self, LU: torch.Tensor, P: torch.Tensor, X: torch.Tensor, dX: torch.Tensor):
+        """Adjoint of X = A^{-1} B from the packed factors and pivots that solve()
+        left: returns (dA packed n_j x n_j, dB packed) for the upstream dX."""
+        dA = torch.empty(max(self.sizeA, 1), dtype=torch.float64, device=self.device)
+        dB = torch.empty(max(self.sizeB, 1), dtype=torch.float64, device=self.device)
+        ptrs = [_pointer_array(LU, self.offA), _pointer_array(P, self.offP), _pointer_array(X, self.offB),
+                _pointer_array(dX, self.offB), _pointer_array(dA, self.offA), _pointer_array(dB, self.offB)]
+        h = handle()
+        st = lib().nrocm_dgetrs_backward_batched(h, self._batch, *[_ptr(t) for t in ptrs])
+        check("nrocm_dgetrs_backward_batched", st, h)
+        return dA[:self.sizeA], dB[:self.sizeB]
+
     def solve_ptrs(self, pa, pp, pb, info, mode: int = EXACT):
         h = handle()
         st = lib().nrocm_dgetrf_getrs_batched(h, self._batch, _ptr(pa), _ptr(pp) if pp is not None else None,

@@ -35,14 +35,13 @@ using nrocm_chol::div_den_ok;
 using nrocm_chol::div_with;
 using nrocm_chol::lane_value;
 using nrocm_chol::recip_refined;
-using nrocm_chol::row_shl;
 
 constexpr int MAXRHS = 4;  // right-hand sides carried through the factor (one per lane row)
 
 template <int R>
 struct TileSmem {
     double colraw[64 + MAXRHS];  // column k before division (rows), then the right-hand sides
-    double colbuf[8][R];         // L_rk at [r % 8][r / 8]
+    double colbuf[8][R + 1];     // L_rk at [r % 8][r / 8] (rows padded: no bank conflicts at R = 8)
     double ycol[MAXRHS];         // coefficient of step k per right-hand side
     double ysave[MAXRHS][64];    // y = L^{-1} b (forward result)
     double dg[64];               // L_kk
@@ -59,6 +58,17 @@ __device__ const double kZeroCellT[2] = {0.0, 1.0};  // a zero, then a one for p
 // barrier drains: each step would wait for its own column store).
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const double gcdouble;
+
+// 64-bit DPP row shift left by N within rows of 16 lanes; lanes whose source
+// leaves the row get an unspecified value (their results are not used).
+template <int N>
+__device__ __forceinline__ double dpp_shl(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, 0x100 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), 0x100 + N, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // The update a = fl(fl((-x) * y) + a) in place (nalgebra's axpy with factor
 // -x), or one fused multiply-add in FAST.  Updating the accumulator register
@@ -242,12 +252,15 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
     double b[MAXRHS];
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) b[c] = (c < nr) ? s.ysave[c][r] : 0.0;
-    // L_ri for the steps i = n-1, n-2, ...: PF of them in flight; rows past
-    // the diagonal and past n read a zero cell
+    // L_ri for the steps i = n-1, n-2, ...: PF of them in flight.  Every lane
+    // loads from a valid address (row min(r, n-1), column max(i, 0)); values of
+    // rows r <= i (the upper triangle) or r >= n only land in product slots
+    // that no dot reads.
     constexpr int PF = 8;
+    const int rl = row_ok ? r : n - 1;
     auto lval = [&](int i) {
-        gcdouble *src = (row_ok && i >= 0 && r >= i) ? A + r + (size_t)i * lda : zc;
-        return *src;
+        gcdouble *col = A + (size_t)(i > 0 ? i : 0) * lda;  // wavefront-uniform
+        return col[rl];
     };
     double cur[PF], nxt[PF];
 #pragma unroll
@@ -266,19 +279,20 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
             lds_barrier();
             const int len = n - i - 1;
             const int full = len & ~7;
-            const double *pc = &s.prod[cq < MAXRHS ? cq : 0][i + 1];
+            const double *pc = &s.prod[cq < MAXRHS ? cq : 0][i + 1 + t];
             double acc = 0.0;
             if (cq < nr) {
-                if constexpr (EXACT) {
-                    // acc_t: positions t, t + 8, ... of the full blocks of 8, in order
-                    for (int e = t; e < full; e += 8) acc += pc[e];
-                } else {
-                    for (int e = t; e < len; e += 8) acc += pc[e];
+                // acc_t: positions t, t + 8, ... of the full blocks of 8 (EXACT)
+                // or of all positions (FAST), in order; a uniform trip count
+                const int nm = EXACT ? (full >> 3) : ((len + 7) >> 3);
+                for (int m = 0; m < nm; ++m) {
+                    const double v = pc[8 * m];
+                    acc += (EXACT || t + 8 * m < len) ? v : 0.0;
                 }
             }
             // lanes t < 4: acc_t + acc_{t+4}; lane t == 0 combines in order
-            const double pair = acc + row_shl<4>(acc);
-            const double p1 = row_shl<1>(pair), p2 = row_shl<2>(pair), p3 = row_shl<3>(pair);
+            const double pair = acc + dpp_shl<4>(acc);
+            const double p1 = dpp_shl<1>(pair), p2 = dpp_shl<2>(pair), p3 = dpp_shl<3>(pair);
             if (cq < nr && t == 0) {
                 double res = 0.0;
                 res += pair;
@@ -286,7 +300,7 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
                 res += p2;
                 res += p3;
                 if constexpr (EXACT)
-                    for (int e = full; e < len; ++e) res += pc[e];  // the tail, in order
+                    for (int e = full; e < len; ++e) res += pc[e];  // the tail, in order (lane t == 0)
                 const double num = s.ysave[cq][i] - res;
                 const double dii = s.dg[i];
                 double xq;

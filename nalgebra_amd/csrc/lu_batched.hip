@@ -197,6 +197,195 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
     }
 }
 
+// Accumulator t of dotx (src/base/blas.rs:93-167) over len products p[0..len)
+// held in LDS: positions t, t + 8, ... of the full blocks of 8, in order.  The
+// caller combines 0 + (acc0+acc4) + (acc1+acc5) + (acc2+acc6) + (acc3+acc7)
+// and adds the tail.
+__device__ __forceinline__ double dotx_acc(const double *p, int len, int t)
+{
+    const int full = len & ~7;
+    double a = 0.0;
+    for (int e = t; e < full; e += 8) a += p[e];
+    return a;
+}
+
+// Adjoint of X = A^{-1} B from LU::new's packed factors and pivots, one
+// member per group of G lanes (lane r holds row r of L \ U):
+//   U^T y = g  (tr_solve_upper_triangular, solve.rs:255-276,411-440),
+//   L^T z = y  (tr_solve_lower_triangular on the unit L, solve.rs:232-253,379-409),
+//   Lambda = P^T z (inv_permute_rows, permutation_sequence.rs:108-116),
+//   dB = Lambda, dA = -Lambda X^T (products summed over right-hand sides in
+//   order), each dot in dotx's order.
+template <int NB, int G>
+__global__ __launch_bounds__(64) void getrs_backward_batched_kernel(
+    const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
+    const double *const *__restrict__ LUptr, const int *const *__restrict__ Pptr,
+    const double *const *__restrict__ Xptr, const double *const *__restrict__ dXptr,
+    double *const *__restrict__ dAptr, double *const *__restrict__ dBptr, int nrhs)
+{
+    constexpr int MPW = 64 / G;
+    __shared__ double prod[MPW][CH][NB];
+    __shared__ double accs[MPW][CH][8];
+    __shared__ double vbuf[MPW][CH][NB];  // z then Lambda, by position; X rows for dA
+    const int lane = threadIdx.x;
+    const int g = lane / G, r = lane % G;
+    const int item = begin + (int)blockIdx.x * MPW + g;
+    const bool valid = item < end;
+    const int jm = valid ? order[item] : 0;
+    const int n = valid ? nsz[jm] : 0;
+    const double *const LU = valid ? LUptr[jm] : nullptr;
+    const int *const P = valid ? Pptr[jm] : nullptr;
+    const double *const X = valid ? Xptr[jm] : nullptr;
+    const double *const dX = valid ? dXptr[jm] : nullptr;
+    double *const dA = valid ? dAptr[jm] : nullptr;
+    double *const dB = valid ? dBptr[jm] : nullptr;
+    const bool row_ok = r < n;
+
+    double a[NB];
+    {
+        const double *row = row_ok ? LU + r : kZeroCellLU;
+        const size_t step = row_ok ? (size_t)n : 0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const double *src = (c < n) ? row + (size_t)c * step : kZeroCellLU;
+            a[c] = *src;
+        }
+    }
+    for (int c0 = 0; c0 < nrhs; c0 += CH) {
+        const int qn = nrhs - c0 < CH ? nrhs - c0 : CH;
+        double y[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) y[q] = (row_ok && q < qn) ? dX[r + (size_t)(c0 + q) * n] : 0.0;
+        // ---- U^T y = g, i ascending: y_i = (g_i - dotx(U[0..i, i], y[0..i])) / U_ii
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const bool live = i < n;
+            if (r < NB) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) prod[g][q][r] = a[i] * y[q];
+            }
+            __syncthreads();
+            if (live && r < 8) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) accs[g][q][r] = dotx_acc(prod[g][q], i, r);
+            }
+            __syncthreads();
+            if (live && r == i) {
+                const int full = i & ~7;
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    const double *ac = accs[g][q];
+                    double res = 0.0;
+                    res += ac[0] + ac[4];
+                    res += ac[1] + ac[5];
+                    res += ac[2] + ac[6];
+                    res += ac[3] + ac[7];
+                    for (int e = full; e < i; ++e) res += prod[g][q][e];
+                    y[q] = (y[q] - res) / a[i];
+                }
+            }
+            __syncthreads();
+        }
+        // ---- L^T z = y, i descending: z_i = (y_i - dotx(L[i+1.., i], z[i+1..])) / 1
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            const bool live = i < n;
+            if (r < NB) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) prod[g][q][r] = a[i] * y[q];
+            }
+            __syncthreads();
+            const int len = n - i - 1;
+            if (live && r < 8) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) accs[g][q][r] = len > 0 ? dotx_acc(prod[g][q] + i + 1, len, r) : 0.0;
+            }
+            __syncthreads();
+            if (live && r == i) {
+                const int full = len > 0 ? (len & ~7) : 0;
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    const double *ac = accs[g][q];
+                    double res = 0.0;
+                    res += ac[0] + ac[4];
+                    res += ac[1] + ac[5];
+                    res += ac[2] + ac[6];
+                    res += ac[3] + ac[7];
+                    for (int e = full; e < len; ++e) res += prod[g][q][i + 1 + e];
+                    y[q] = (y[q] - res) / 1.0;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- Lambda = P^T z: the swaps (i, ipiv[i]) in reverse order
+        if (row_ok) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) vbuf[g][q][r] = y[q];
+        }
+        __syncthreads();
+        if (valid && r == 0) {
+            for (int i = n - 1; i >= 0; --i) {
+                const int pv = P[i];
+                if (pv != i) {
+#pragma unroll
+                    for (int q = 0; q < CH; ++q) {
+                        const double t = vbuf[g][q][i];
+                        vbuf[g][q][i] = vbuf[g][q][pv];
+                        vbuf[g][q][pv] = t;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        double lam[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) lam[q] = row_ok ? vbuf[g][q][r] : 0.0;
+        if (row_ok) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+                if (q < qn) dB[r + (size_t)(c0 + q) * n] = lam[q];
+        }
+        __syncthreads();
+        // ---- dA[r, c] = -(sum over right-hand sides of fl(Lambda_r * X_c)), X rows through LDS
+        if (row_ok) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) vbuf[g][q][r] = (q < qn) ? X[r + (size_t)(c0 + q) * n] : 0.0;
+        }
+        __syncthreads();
+        if (row_ok) {
+            for (int c = 0; c < n; ++c) {
+                double sacc = c0 > 0 ? -dA[r + (size_t)c * n] : 0.0;
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    if (q < qn) {
+                        const double pr = lam[q] * vbuf[g][q][c];
+                        sacc = (c0 == 0 && q == 0) ? pr : sacc + pr;
+                    }
+                }
+                dA[r + (size_t)c * n] = -sacc;
+            }
+        }
+        __syncthreads();
+    }
+    // no right-hand sides: X and its gradient are empty
+    if (nrhs == 0 && row_ok)
+        for (int c = 0; c < n; ++c) dA[r + (size_t)c * n] = 0.0;
+}
+
+template <int NB>
+int launch_bwd_nb(hipStream_t st, const int *order, int begin, int end, const int *nsz, const double *const *LU,
+                  const int *const *P, const double *const *X, const double *const *dX, double *const *dA,
+                  double *const *dB, int nrhs)
+{
+    constexpr int G = group_for<NB>();
+    constexpr int MPW = 64 / G;
+    const unsigned grid = (unsigned)((end - begin + MPW - 1) / MPW);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL((getrs_backward_batched_kernel<NB, G>), dim3(grid), dim3(64), 0, st, order, begin, end, nsz,
+                       LU, P, X, dX, dA, dB, nrhs);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 template <int NB>
 int launch_nb(bool exact, hipStream_t st, const int *order, int begin, int end, const int *nsz, double *const *A,
               int *const *P, double *const *B, int nrhs, int *info)
@@ -218,6 +407,10 @@ using Launch = int (*)(bool, hipStream_t, const int *, int, int, const int *, do
                        double *const *, int, int *);
 const Launch kLaunch[8] = {launch_nb<8>, launch_nb<16>, launch_nb<24>, launch_nb<32>,
                            launch_nb<40>, launch_nb<48>, launch_nb<56>, launch_nb<64>};
+using BwdLaunch = int (*)(hipStream_t, const int *, int, int, const int *, const double *const *, const int *const *,
+                          const double *const *, const double *const *, double *const *, double *const *, int);
+const BwdLaunch kBwdLaunch[8] = {launch_bwd_nb<8>,  launch_bwd_nb<16>, launch_bwd_nb<24>, launch_bwd_nb<32>,
+                                 launch_bwd_nb<40>, launch_bwd_nb<48>, launch_bwd_nb<56>, launch_bwd_nb<64>};
 
 }  // namespace
 
@@ -236,6 +429,29 @@ extern "C" int nrocm_dgetrf_getrs_batched(nrocm_handle h, nrocm_batch bt, double
         if (bin.end <= bin.begin) continue;
         if (kLaunch[bin.kernel](mode == NROCM_EXACT, h->stream, bt->order_dev, bin.begin, bin.end, bt->n_dev, a, ipiv,
                                 bt->nrhs > 0 ? b : nullptr, bt->nrhs, info))
+            NROCM_CHECK_LAUNCH(h);
+    }
+    return NROCM_OK;
+}
+
+extern "C" int nrocm_dgetrs_backward_batched(nrocm_handle h, nrocm_batch bt, const double *const *lu,
+                                             const int *const *ipiv, const double *const *x,
+                                             const double *const *dx, double *const *da, double *const *db)
+{
+    if (!h) return -1;
+    if (!bt || bt->handle != h) return -2;
+    if (bt->count > 0 && !lu) return -3;
+    if (bt->count > 0 && !ipiv) return -4;
+    if (bt->count > 0 && bt->nrhs > 0 && !x) return -5;
+    if (bt->count > 0 && bt->nrhs > 0 && !dx) return -6;
+    if (bt->count > 0 && !da) return -7;
+    if (bt->count > 0 && bt->nrhs > 0 && !db) return -8;
+    if (bt->count == 0) return NROCM_OK;
+    if (bt->max_n > 64) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "batched LU adjoint supports n <= 64");
+    for (const auto &bin : bt->bins) {
+        if (bin.end <= bin.begin) continue;
+        if (kBwdLaunch[bin.kernel](h->stream, bt->order_dev, bin.begin, bin.end, bt->n_dev, lu, ipiv, x, dx, da, db,
+                                   bt->nrhs))
             NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;

@@ -57,6 +57,57 @@ void nro_d_potrs_adjoint(long n, const double *L, long lda, const double *x,
         }
 }
 
+/* Adjoint of X = A^{-1} B for A = P^{-1} L U (LU::new's packed factors and
+ * pivots), upstream gradient G (n x nrhs):
+ *   Lambda = A^{-T} G = P^T L^{-T} U^{-T} G, dB = Lambda, dA = -Lambda X^T.
+ * Composed from nalgebra's own routines as a user would write it:
+ *   U^T y = g : tr_solve_upper_triangular_mut on U (solve.rs:255-276, :411-440:
+ *               y_i = (g_i - dotx(U[0..i, i], y[0..i])) / U_ii, i ascending);
+ *   L^T z = y : tr_solve_lower_triangular_mut on the unit lower L = lu.l()
+ *               (solve.rs:232-253, :379-409: z_i = (y_i - dotx(L[i+1.., i], z[i+1..])) / 1,
+ *               i descending);
+ *   Lambda    : inv_permute_rows (permutation_sequence.rs:108-116: the swaps in
+ *               reverse order).
+ * dA[r, c] = -(sum over right-hand sides q, in order, of fl(Lambda[r, q] * X[c, q]))
+ * (one product for one right-hand side).  Returns 0 when U has an exact zero
+ * on its diagonal (tr_solve returns false), else 1. */
+int nro_d_getrs_adjoint(long n, long nrhs, const double *LU, long lda, const int *ipiv, const double *X,
+                        long ldx, const double *G, long ldg, double *dA, long ldda, double *dB, long lddb)
+{
+    for (long q = 0; q < nrhs; ++q) {
+        double *y = dB + q * lddb;
+        for (long i = 0; i < n; ++i) y[i] = G[i + q * ldg];
+        for (long i = 0; i < n; ++i) {
+            const double dot = nro_d_dot(i, LU + i * lda, 1, y, 1);
+            const double d = LU[i + i * lda];
+            if (d == 0.0) return 0;
+            y[i] = (y[i] - dot) / d;
+        }
+        for (long i = n - 1; i >= 0; --i) {
+            const double dot = nro_d_dot(n - i - 1, LU + (i + 1) + i * lda, 1, y + i + 1, 1);
+            y[i] = (y[i] - dot) / 1.0;
+        }
+        for (long i = n - 1; i >= 0; --i) {
+            const long p = ipiv[i];
+            if (p != i) {
+                const double t = y[i];
+                y[i] = y[p];
+                y[p] = t;
+            }
+        }
+    }
+    for (long c = 0; c < n; ++c)
+        for (long r = 0; r < n; ++r) {
+            double s = 0.0;
+            for (long q = 0; q < nrhs; ++q) {
+                const double pr = dB[r + q * lddb] * X[c + q * ldx];
+                s = q == 0 ? pr : s + pr;
+            }
+            dA[r + c * ldda] = -s;
+        }
+    return 1;
+}
+
 /* Batched Cholesky factor + solve over matrices stored at the given element
  * offsets (column-major, lda = n, B is n x nrhs with ldb = n).  This is the
  * per-matrix loop a nalgebra user would write: Cholesky::new then solve_mut. */

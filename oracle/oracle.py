@@ -62,6 +62,8 @@ def lib():
             f = getattr(_lib, f"nro_{suf}_qr_ls"); f.restype = C.c_int; f.argtypes = [_L, _L, P, _L, P, P, P]
         _lib.nro_d_potrs_adjoint.restype = None
         _lib.nro_d_potrs_adjoint.argtypes = [_L, _dp, _L, _dp, _dp, _dp, _L, _dp]
+        _lib.nro_d_getrs_adjoint.restype = C.c_int
+        _lib.nro_d_getrs_adjoint.argtypes = [_L, _L, _dp, _L, _ip, _dp, _L, _dp, _L, _dp, _L, _dp, _L]
         _lib.nro_d_potrf_potrs_batched.restype = None
         _lib.nro_d_potrf_potrs_batched.argtypes = [_L, _ip, _i64p, _dp, _i64p, _dp, _L, _ip]
         _lib.nro_s_qr_ls_batched.restype = None
@@ -188,6 +190,20 @@ def potrs_adjoint(L, x, g):
     dA = np.zeros((n, n), order="F"); db = np.zeros(n)
     lib().nro_d_potrs_adjoint(n, _ptr(L), n, _ptr(x), _ptr(g), _ptr(dA), n, _ptr(db))
     return dA, db
+
+
+def getrs_adjoint(LU, ipiv, X, G):
+    """Adjoint of X = A^{-1} B from LU::new's factors: (dA, dB, ok) for the
+    upstream gradient G = dLoss/dX (n x nrhs): dB = A^{-T} G, dA = -dB X^T."""
+    LU = _fmat(LU, np.float64); n = LU.shape[0]
+    X = _fmat(np.asarray(X, dtype=np.float64).reshape(n, -1), np.float64)
+    G = _fmat(np.asarray(G, dtype=np.float64).reshape(n, -1), np.float64)
+    nrhs = X.shape[1]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    dA = np.zeros((n, n), order="F"); dB = np.zeros((n, nrhs), order="F")
+    ok = lib().nro_d_getrs_adjoint(n, nrhs, _ptr(LU), max(n, 1), ipiv.ctypes.data_as(_ip), _ptr(X), max(n, 1),
+                                   _ptr(G), max(n, 1), _ptr(dA), max(n, 1), _ptr(dB), max(n, 1))
+    return dA, dB, bool(ok)
 
 
 # ---------------------------------------------------------------- QR

@@ -134,3 +134,66 @@ def test_batched_lu_many_members(nl, orc):
     for j, n in enumerate(ns):
         assert np.array_equal(Pg[bt.offP[j]:bt.offP[j] + n], Ps[j])
         assert np.array_equal(_bits(Bg[offB[j]:offB[j] + 4 * n]), _bits(Xs[j].ravel(order="F")))
+
+
+@pytest.mark.parametrize("nrhs", [1, 4, 6, 0])
+def test_batched_lu_adjoint_bitwise(nl, orc, nrhs):
+    """The LU-solve adjoint (dB = A^{-T} dX, dA = -dB X^T) is bit-identical to
+    the oracle's composition of nalgebra's transposed solves and
+    inv_permute_rows on every member."""
+    ns = [1, 2, 7, 8, 9, 16, 23, 31, 32, 33, 40, 47, 48, 55, 63, 64, 64, 5]
+    ns, A, offA, B, offB = make_batch(ns, nrhs, 71 + nrhs)
+    bt, Ag, Bg, Pg, info = run(nl, ns, A, B, nrhs, nl.EXACT)
+    assert not info.any()
+    dX = np.zeros_like(B)
+    for j, n in enumerate(ns):
+        if n and nrhs:
+            dX[offB[j]:offB[j] + n * nrhs] = gen.normal(900 + j, (n * nrhs,))
+    LUd = torch.from_numpy(Ag).cuda(); Pd = torch.from_numpy(Pg).cuda()
+    Xd = torch.from_numpy(Bg).cuda(); dXd = torch.from_numpy(dX if dX.size else np.zeros(1)).cuda()
+    dA, dB = bt.backward(LUd, Pd, Xd, dXd)
+    dA = dA.cpu().numpy(); dB = dB.cpu().numpy()
+    for j, n in enumerate(ns):
+        LU = Ag[offA[j]:offA[j] + n * n].reshape(n, n, order="F")
+        X = Bg[offB[j]:offB[j] + n * nrhs].reshape(n, nrhs, order="F")
+        G = dX[offB[j]:offB[j] + n * nrhs].reshape(n, nrhs, order="F")
+        if nrhs == 0:
+            assert not dA[offA[j]:offA[j] + n * n].any()
+            continue
+        dAr, dBr, ok = orc.getrs_adjoint(LU, Pg[bt.offP[j]:bt.offP[j] + n], X, G)
+        assert ok
+        assert np.array_equal(_bits(dB[offB[j]:offB[j] + n * nrhs]), _bits(dBr.ravel(order="F"))), (j, n)
+        assert np.array_equal(_bits(dA[offA[j]:offA[j] + n * n]), _bits(dAr.ravel(order="F"))), (j, n)
+
+
+def test_batched_lu_adjoint_finite_differences(nl):
+    """Central differences of loss = sum(g * x), x = A^{-1} b, in entries of A
+    and b (h = 1e-6, relative 1e-6), through the GPU forward and adjoint."""
+    rng = np.random.default_rng(81)
+    n = 12
+    A0 = rng.standard_normal((n, n)) + 2 * np.eye(n)
+    b0 = rng.standard_normal(n); g = rng.standard_normal(n)
+
+    def fwd(Am, bv):
+        bt = nl.LUBatch([n], 1)
+        Ad = torch.from_numpy(Am.ravel(order="F").copy()).cuda()
+        Bd = torch.from_numpy(bv.copy()).cuda()
+        P, info = bt.solve(Ad, Bd, nl.EXACT)
+        return bt, Ad, P, Bd
+
+    bt, LUd, Pd, Xd = fwd(A0, b0)
+    dA, dB = bt.backward(LUd, Pd, Xd, torch.from_numpy(g).cuda())
+    dA = dA.cpu().numpy().reshape(n, n, order="F"); dB = dB.cpu().numpy()
+
+    def loss(Am, bv):
+        return float(np.dot(g, fwd(Am, bv)[3].cpu().numpy()))
+
+    h = 1e-6
+    for (i, j) in [(0, 0), (3, 7), (11, 2), (5, 5), (8, 11)]:
+        Ap = A0.copy(); Am = A0.copy(); Ap[i, j] += h; Am[i, j] -= h
+        fd = (loss(Ap, b0) - loss(Am, b0)) / (2 * h)
+        assert abs(fd - dA[i, j]) <= 1e-6 * max(1.0, abs(dA[i, j]))
+    for i in (0, 6, 11):
+        bp = b0.copy(); bm = b0.copy(); bp[i] += h; bm[i] -= h
+        fd = (loss(A0, bp) - loss(A0, bm)) / (2 * h)
+        assert abs(fd - dB[i]) <= 1e-6 * max(1.0, abs(dB[i]))

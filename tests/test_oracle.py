@@ -366,3 +366,82 @@ def test_adjoint_closed_form_and_fd(orc):
         bp = b.copy(); bm = b.copy(); bp[i] += h; bm[i] -= h
         fd = (loss(A, bp)[0] - loss(A, bm)[0]) / (2 * h)
         assert abs(fd - db[i]) <= 1e-6 * max(1.0, abs(db[i]))
+
+
+def test_lu_adjoint_closed_form_and_fd(orc):
+    """Adjoint of x = A^{-1} b through LU::new's factors: dB = A^{-T} g and
+    dA = -dB x^T against the closed form (numpy, 1e-12) and central finite
+    differences of loss = sum(g * x) in every entry of A and b."""
+    rng = np.random.default_rng(14)
+    for n, nrhs in [(1, 1), (5, 2), (9, 1), (33, 3)]:
+        A = rng.standard_normal((n, n)) + 0.5 * np.eye(n)
+        B = rng.standard_normal((n, nrhs)); g = rng.standard_normal((n, nrhs))
+
+        def loss(Am, Bm):
+            LU, ip, _ = orc.getrf(Am)
+            X, ok = orc.getrs(LU, ip, Bm)
+            assert ok
+            return float(np.sum(g * X)), LU, ip, X
+
+        f0, LU, ip, X = loss(A, B)
+        dA, dB, ok = orc.getrs_adjoint(LU, ip, X, g)
+        assert ok
+        lam = np.linalg.solve(A.T, g)
+        assert np.allclose(dB, lam, rtol=1e-12, atol=1e-13)
+        assert np.allclose(dA, -lam @ X.T, rtol=1e-12, atol=1e-13)
+        if n <= 9:
+            h = 1e-6
+            for i in range(n):
+                for j in range(n):
+                    Ap = A.copy(); Am = A.copy(); Ap[i, j] += h; Am[i, j] -= h
+                    fd = (loss(Ap, B)[0] - loss(Am, B)[0]) / (2 * h)
+                    assert abs(fd - dA[i, j]) <= 1e-6 * max(1.0, abs(dA[i, j]))
+                for q in range(nrhs):
+                    Bp = B.copy(); Bm = B.copy(); Bp[i, q] += h; Bm[i, q] -= h
+                    fd = (loss(A, Bp)[0] - loss(A, Bm)[0]) / (2 * h)
+                    assert abs(fd - dB[i, q]) <= 1e-6 * max(1.0, abs(dB[i, q]))
+
+
+def test_lu_adjoint_exact_dyadic(orc):
+    """Factors whose every operation is exact (small integers, unit L, U
+    diagonal of +-1 and +-2, pivots ipiv[i] >= i): the adjoint must equal the
+    exact rational result bit for bit, which pins the transposed solves'
+    algebra and the direction of the permutation (P^T, swaps in reverse)."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(15)
+    n = 6
+    L = np.tril(rng.integers(-1, 2, (n, n)).astype(float), -1) + np.eye(n)
+    U = np.triu(rng.integers(-2, 3, (n, n)).astype(float), 1)
+    U[np.arange(n), np.arange(n)] = [1.0, -2.0, 2.0, -1.0, 2.0, 1.0]
+    LU = np.tril(L, -1) + U
+    ipiv = np.array([2, 1, 5, 4, 4, 5], dtype=np.int32)
+    X = rng.integers(-3, 4, (n, 2)).astype(float)
+    g = rng.integers(-3, 4, (n, 2)).astype(float)
+    dA, dB, ok = orc.getrs_adjoint(LU, ipiv, X, g)
+    assert ok
+    # exact: A = P^{-1} L U with P = swaps (i, ipiv[i]) applied in order
+    P = np.eye(n)
+    for i, p in enumerate(ipiv):
+        P[[i, p]] = P[[p, i]]
+    A = np.linalg.inv(P) @ L @ U
+    Af = [[Fr(int(round(v))) for v in row] for row in A]
+    # solve A^T lam = g exactly by Gauss-Jordan on fractions
+    for q in range(2):
+        M = [[Af[j][i] for j in range(n)] + [Fr(int(g[i, q]))] for i in range(n)]
+        for c in range(n):
+            piv = next(r for r in range(c, n) if M[r][c] != 0)
+            M[c], M[piv] = M[piv], M[c]
+            M[c] = [v / M[c][c] for v in M[c]]
+            for r in range(n):
+                if r != c and M[r][c] != 0:
+                    M[r] = [a - M[r][c] * b for a, b in zip(M[r], M[c])]
+        lam = [M[i][n] for i in range(n)]
+        assert [Fr(v) for v in dB[:, q]] == lam
+    ref = -(dB @ X.T)
+    assert np.array_equal(dA, ref)
+
+
+def test_lu_adjoint_singular(orc):
+    LU = np.triu(np.ones((4, 4))); LU[2, 2] = 0.0
+    _, _, ok = orc.getrs_adjoint(LU, np.arange(4, dtype=np.int32), np.ones((4, 1)), np.ones((4, 1)))
+    assert not ok
