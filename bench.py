@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the accelerated nalgebra path on one MI355X.
 
-Default workload (the batched-Cholesky half of the headline metric):
-65,536 f64 SPD systems, n_j uniform integer in [4, 64], A_j = G G^T + n_j I
-with G ~ N(0,1), 4 right-hand sides ~ N(0,1); every member is factored
-(Cholesky::new) and solved (Cholesky::solve_mut) in one hipGraph replay of
-the batched kernels.  A step is one such pass over a fresh, device-resident
-copy of the batch (the factorisation is in place, so each timed step gets its
-own copy, prepared before the timed region).
+Default run (`python bench.py`): one JSON line whose headline is the batched
+Cholesky half of the metric, with the other workloads attached as records:
 
-    python bench.py [--steps K] [--warmup W] [--mode exact|fast]
-                    [--workload chol|lu] [--no-cpu]
+  headline       65,536 f64 SPD systems, n_j uniform integer in [4, 64],
+                 A_j = G G^T + n_j I (G ~ N(0,1)), 4 right-hand sides ~ N(0,1);
+                 Cholesky::new + solve_mut of every member (NROCM_EXACT, bit
+                 for bit) in one hipGraph replay.  A step is one pass over a
+                 fresh device-resident copy (prepared before the timed region).
+  cholesky_fast  the same batch in NROCM_FAST (within 1e-12 of the oracle).
+  lu_mixed_ir    n = 8192, kappa = 1e4: GPU FP64 LU (partial and windowed
+                 pivoting) against FP32 / split-BF16 trailing updates + FP64
+                 iterative refinement, all to backward error <= 1e-14.
+  cholesky_adjoint, least_squares, lu_batched: the differentiated, least-
+                 squares and general batched workloads.
 
-Prints one JSON line.  `value` is matrices/s; `roofline` prices the step
-against HBM (algorithmic bytes: read the lower triangle of A and B, write L
-and X); `cpu_baseline` times the CPU oracle (a C restatement of nalgebra's
-Cholesky::new + solve_mut, one thread) on the same batch.
+    python bench.py [--workload all|chol|lu|adjoint|lsq|lu_batched]
+                    [--steps K] [--warmup W] [--mode exact|fast] [--no-cpu]
+
+`value` is matrices/s (the headline); every record carries its `roofline`
+(algorithmic bytes or flops against the MI355X peak) and a `cpu_baseline`
+timed with the oracle (a C restatement of nalgebra's loops) on this host.
 """
 from __future__ import annotations
 
@@ -61,7 +67,7 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--mode", choices=["exact", "fast"], default="exact")
-    p.add_argument("--workload", choices=["all", "chol", "lu", "adjoint", "lsq"], default="all")
+    p.add_argument("--workload", choices=["all", "chol", "lu", "adjoint", "lsq", "lu_batched"], default="all")
     p.add_argument("--count", type=int, default=65536)
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
@@ -286,6 +292,9 @@ def bench_lu(args):
     if args.lu_variants:
         keep = set(args.lu_variants.split(",")) | {"fp64"}
         precs = [(k, v) for k, v in precs if k in keep]
+    A2 = A.view(n, n).T  # A[i, j] as a row-major view of the column-major storage
+    nrmA = float(A2.abs().sum(dim=1).max())
+    xs = {}
     for name, p in precs:
         times, iters, berr, ok = [], 0, 0.0, False
         for rep in range(args.warmup_lu + args.reps_lu):
@@ -296,8 +305,17 @@ def bench_lu(args):
             if rep >= args.warmup_lu:
                 times.append((time.perf_counter() - t0) * 1e3)
         ms = float(np.median(times))
-        res[name] = {"ms": ms, "iters": iters, "berr": berr, "converged": ok,
+        xv = x.data.view(-1)
+        xs[name] = xv
+        # normwise backward error recomputed here in FP64
+        eta = float((b - A2 @ xv).abs().max() / (nrmA * xv.abs().max() + b.abs().max()))
+        res[name] = {"ms": ms, "iters": iters, "berr": berr, "berr_recomputed": eta, "converged": ok,
+                     "fell_back_to_partial_pivoting": bool(nl.last_ir_fallback()),
                      "factor_tflops_equiv": flops / (ms * 1e-3) / 1e12}
+    # forward error of every variant against the GPU FP64 LU (partial pivoting) solution
+    ref = xs["fp64"]
+    for name in res:
+        res[name]["forward_err_vs_fp64"] = float((xs[name] - ref).abs().max() / ref.abs().max())
     # speedups against the GPU FP64 LU with partial pivoting, and for the
     # windowed variants also against FP64 with the same pivoting
     t64 = res["fp64"]["ms"]
@@ -307,12 +325,18 @@ def bench_lu(args):
         if "_window" in name and ref.get("converged"):
             res[name]["speedup_vs_fp64_window"] = ref["ms"] / res[name]["ms"]
     best = min((v["ms"], k) for k, v in res.items() if v["converged"] and not k.startswith("fp64"))
+    for name, v in res.items():
+        if v["converged"] and not v["berr_recomputed"] <= 2e-14:
+            raise SystemExit(f"LU variant {name}: recomputed backward error {v['berr_recomputed']:.3e} > 2e-14")
     out = {
         "metric": METRIC, "value": best[0], "unit": "ms", "n_gpus": 1, "steps": args.reps_lu,
         "warmup": args.warmup_lu, "ms_per_step": best[0], "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "lu_mixed_ir", "n": n, "kappa": 1e4, "tol": 1e-14, "best": best[1],
                    "seed": args.seed},
+        "speedup_vs_fp64_partial_pivoting": res["fp64"]["ms"] / best[0],
+        "speedup_vs_fp64_same_pivoting": (res["fp64_window"]["ms"] / best[0]
+                                          if "_window" in best[1] and "fp64_window" in res else None),
         "results": res,
         "roofline": {"bound": "mfma", "achieved": res[best[1]]["factor_tflops_equiv"],
                      "peak": FP64_PEAK_TFLOPS if best[1].startswith("fp64") else FP32_PEAK_TFLOPS,
@@ -335,10 +359,94 @@ def bench_lu(args):
         tc = time.perf_counter() - t0
         est = tc * (n / ncpu) ** 3
         out["cpu_baseline"] = {"value": est * 1e3, "unit": "ms" if ncpu == n else "ms (extrapolated)", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", **host_info(),
                                "sample": (f"oracle LU::new of the same n={n} matrix, {tc:.1f} s" if ncpu == n else
                                           f"oracle LU::new of the leading {ncpu}x{ncpu} block took {tc:.2f} s; "
                                           f"scaled by (n/{ncpu})^3")}
+    return out
+
+
+def bench_lu_batched(args):
+    """Batched uneven general systems: 65,536 f64 matrices, n_j uniform in
+    [4, 64], entries uniform[-1, 1], 4 right-hand sides; one step = LU::new +
+    LU::solve_mut of every member (nrocm_dgetrf_getrs_batched, EXACT) on a fresh
+    device-resident copy.  value = matrices/s."""
+    import torch
+    import nalgebra_amd as nl
+    from nalgebra_amd import _lib
+    from nalgebra_amd.batched import _pointer_array
+    lib = _lib.lib()
+    nrhs, count = 4, args.count
+    ns = draw_sizes(lib, args.seed + 100, count)
+    n64 = ns.astype(np.int64)
+    offA = np.concatenate([[0], np.cumsum(n64 * n64)[:-1]]).astype(np.int64)
+    offB = np.concatenate([[0], np.cumsum(n64 * nrhs)[:-1]]).astype(np.int64)
+    offP = np.concatenate([[0], np.cumsum(n64)[:-1]]).astype(np.int64)
+    A = np.zeros(int((n64 * n64).sum())); B = np.zeros(int((n64 * nrhs).sum()))
+    lib.nrocm_gen_uniform(args.seed + 101, A.size, -1.0, 1.0, A.ctypes.data_as(C.c_void_p))
+    lib.nrocm_gen_uniform(args.seed + 102, B.size, -1.0, 1.0, B.ctypes.data_as(C.c_void_p))
+    dev = torch.device("cuda:0")
+    A0 = torch.from_numpy(A).to(dev); B0 = torch.from_numpy(B).to(dev)
+    K = max(1, args.steps)
+    copies = [(A0.clone(), B0.clone()) for _ in range(K)]
+    bt = nl.LUBatch(ns, nrhs, device=dev)
+    P = torch.empty(int(n64.sum()), dtype=torch.int32, device=dev)
+    info = torch.empty(count, dtype=torch.int32, device=dev)
+    ptrs = [(_pointer_array(a, offA), _pointer_array(P, offP), _pointer_array(b, offB)) for a, b in copies]
+    mode = nl.EXACT if args.mode == "exact" else nl.FAST
+    for i in range(max(args.warmup, 1)):
+        bt.solve_ptrs(*ptrs[i % K], info, mode)
+    torch.cuda.synchronize()
+    for a, b in copies:
+        a.copy_(A0); b.copy_(B0)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for i in range(K):
+        bt.solve_ptrs(*ptrs[i], info, mode)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    # parity on a sample of members against the oracle's LU::new + solve_mut
+    from oracle import oracle as orc
+    sample = np.arange(0, count, max(1, count // 512))
+    Ar, Br = A.copy(), B.copy()
+    pr, infr = orc.getrf_getrs_batched(ns[sample], Ar, offA[sample], Br, offB[sample],
+                                       np.concatenate([[0], np.cumsum(n64[sample])[:-1]]), nrhs)
+    Bg = copies[-1][1].cpu().numpy(); Pg = P.cpu().numpy()
+    bit, piv_ok, po = True, True, 0
+    for k, j in enumerate(sample):
+        n = ns[j]
+        sl = slice(offB[j], offB[j] + nrhs * n)
+        bit &= bool(np.array_equal(Bg[sl].view(np.uint64), Br[sl].view(np.uint64)))
+        piv_ok &= bool(np.array_equal(Pg[offP[j]:offP[j] + n], pr[po:po + n]))
+        po += n
+    if args.mode == "exact" and not (bit and piv_ok):
+        raise SystemExit("EXACT batched LU is not bit-identical to the oracle on the sampled members")
+    alg_bytes = int((8 * (2 * n64 * n64 + 2 * nrhs * n64) + 4 * n64).sum())
+    out = {"metric": METRIC, "value": count / (ms * 1e-3), "unit": "matrices/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "batched_lu_solve", "count": count, "n_range": [4, 64], "nrhs": nrhs,
+                      "mode": args.mode, "seed": args.seed + 100},
+           "roofline": {"bound": "hbm", "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "alg_bytes_per_step": alg_bytes,
+                        "note": "read A, write the packed L\\U, read B, write X, write the pivots"},
+           "parity": {"sampled_members": int(len(sample)), "bit_identical": bit, "pivots_identical": piv_ok}}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < args.cpu_seconds and reps < 100:
+            Aw, Bw = A.copy(), B.copy()
+            orc.getrf_getrs_batched(ns, Aw, offA, Bw, offB, offP, nrhs)
+            reps += 1
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": reps * count / tc, "unit": "matrices/s", "cores": 1, "kind": "port",
+                               **host_info(),
+                               "sample": f"the full {count}-member batch x{reps} through the oracle's LU::new + "
+                                         f"solve_mut (one thread), {tc:.1f} s"}
     return out
 
 
@@ -418,7 +526,7 @@ def bench_adjoint(args):
                     break
         tc = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": reps / tc, "unit": "matrices/s (forward+backward)", "cores": 1,
-                               "kind": "port", "sample": f"{reps} members through the oracle (ctypes per call), "
+                               "kind": "port", **host_info(), "sample": f"{reps} members through the oracle (ctypes per call), "
                                                          f"{tc:.1f} s"}
     return out
 
@@ -488,7 +596,7 @@ def bench_lsq(args):
                       b32[offb[j]:offb[j] + ms_[j]])
             done += 1
         tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": done / tc, "unit": "systems/s", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": done / tc, "unit": "systems/s", "cores": 1, "kind": "port", **host_info(),
                                "sample": f"{done} systems (the batch in order, repeated) through the oracle's f32 "
                                          f"QR::new + q_tr_mul + upper solve, {tc:.1f} s"}
     return out
@@ -516,11 +624,17 @@ def main():
         sub.seed = args.seed
         out["cholesky_adjoint"] = bench_adjoint(sub)
         out["least_squares"] = bench_lsq(sub)
-        for k in ("lu_mixed_ir", "cholesky_adjoint", "least_squares"):
+        out["lu_batched"] = bench_lu_batched(sub)
+        fast = argparse.Namespace(**vars(args))
+        fast.mode, fast.no_cpu = "fast", True
+        out["cholesky_fast"] = bench_chol(fast)
+        for k in ("lu_mixed_ir", "cholesky_adjoint", "least_squares", "lu_batched", "cholesky_fast"):
             for f in ("metric", "n_gpus", "scaling", "vs_baseline", "data"):
                 out[k].pop(f, None)
     elif args.workload == "chol":
         out = bench_chol(args)
+    elif args.workload == "lu_batched":
+        out = bench_lu_batched(args)
     elif args.workload == "lu":
         out = bench_lu(args)
     elif args.workload == "adjoint":

@@ -124,6 +124,27 @@ void nro_d_potrf_potrs_batched(long count, const int *ns, const int64_t *offA, d
     }
 }
 
+/* Batched LU::new + LU::solve_mut over matrices at the given element offsets
+ * (column-major, lda = n; B is n x nrhs, ldb = n): the per-matrix loop a
+ * nalgebra user would write.  ipiv receives each member's pivots at offP[j];
+ * info[j] = 0, or 1 + the largest i with U_ii == 0 (solve_mut returns false;
+ * B_j is then left as it was). */
+void nro_d_getrf_getrs_batched(long count, const int *ns, const int64_t *offA, double *Abase, const int64_t *offB,
+                               double *Bbase, const int64_t *offP, int *ipiv, long nrhs, int *info)
+{
+    for (long j = 0; j < count; ++j) {
+        const long n = ns[j];
+        double *A = Abase + offA[j];
+        int *piv = ipiv + offP[j];
+        nro_d_getrf(n, n, A, n, piv);
+        long z = 0;
+        for (long i = 0; i < n; ++i)
+            if (A[i + i * n] == 0.0) z = i + 1;
+        info[j] = (int)z;
+        if (z == 0 && nrhs > 0) nro_d_getrs(n, nrhs, A, n, piv, Bbase + offB[j], n);
+    }
+}
+
 /* Batched tall least squares in f32 (QR::new + q_tr_mul + upper solve). */
 void nro_s_qr_ls_batched(long count, const int *ms, long n, const int64_t *offA, float *Abase,
                          const int64_t *offb, float *bbase, float *diag, float *x, int *info)

@@ -64,6 +64,8 @@ def lib():
         _lib.nro_d_potrs_adjoint.argtypes = [_L, _dp, _L, _dp, _dp, _dp, _L, _dp]
         _lib.nro_d_getrs_adjoint.restype = C.c_int
         _lib.nro_d_getrs_adjoint.argtypes = [_L, _L, _dp, _L, _ip, _dp, _L, _dp, _L, _dp, _L, _dp, _L]
+        _lib.nro_d_getrf_getrs_batched.restype = None
+        _lib.nro_d_getrf_getrs_batched.argtypes = [_L, _ip, _i64p, _dp, _i64p, _dp, _i64p, _ip, _L, _ip]
         _lib.nro_d_potrf_potrs_batched.restype = None
         _lib.nro_d_potrf_potrs_batched.argtypes = [_L, _ip, _i64p, _dp, _i64p, _dp, _L, _ip]
         _lib.nro_s_qr_ls_batched.restype = None
@@ -284,6 +286,19 @@ def potrf_potrs_batched(ns, A_flat, offA, B_flat, offB, nrhs):
                                     _ptr(A_flat), offB.ctypes.data_as(_i64p), _ptr(B_flat),
                                     nrhs, info.ctypes.data_as(_ip))
     return info
+
+
+def getrf_getrs_batched(ns, A_flat, offA, B_flat, offB, offP, nrhs):
+    """Per-member LU::new + solve_mut in place; returns (pivots packed, info)."""
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    offA = np.ascontiguousarray(offA, dtype=np.int64); offB = np.ascontiguousarray(offB, dtype=np.int64)
+    offP = np.ascontiguousarray(offP, dtype=np.int64)
+    piv = np.zeros(max(int(ns.astype(np.int64).sum()), 1), dtype=np.int32)
+    info = np.zeros(len(ns), dtype=np.int32)
+    lib().nro_d_getrf_getrs_batched(len(ns), ns.ctypes.data_as(_ip), offA.ctypes.data_as(_i64p), _ptr(A_flat),
+                                    offB.ctypes.data_as(_i64p), _ptr(B_flat), offP.ctypes.data_as(_i64p),
+                                    piv.ctypes.data_as(_ip), nrhs, info.ctypes.data_as(_ip))
+    return piv, info
 
 
 def qr_ls_batched(ms, n, A_flat, offA, b_flat, offb):
