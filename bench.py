@@ -15,8 +15,9 @@ Cholesky half of the metric, with the other workloads attached as records:
                  iterative refinement, all to backward error <= 1e-14.
   cholesky_adjoint, least_squares, lu_batched: the differentiated, least-
                  squares and general batched workloads.
+  gemm           Matrix::gemm at 512^3 and 8192^3 (FP64 and FP32 MFMA).
 
-    python bench.py [--workload all|chol|lu|adjoint|lsq|lu_batched]
+    python bench.py [--workload all|chol|lu|adjoint|lsq|lu_batched|gemm]
                     [--steps K] [--warmup W] [--mode exact|fast] [--no-cpu]
 
 `value` is matrices/s (the headline); every record carries its `roofline`
@@ -67,7 +68,8 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--mode", choices=["exact", "fast"], default="exact")
-    p.add_argument("--workload", choices=["all", "chol", "lu", "adjoint", "lsq", "lu_batched"], default="all")
+    p.add_argument("--workload", choices=["all", "chol", "lu", "adjoint", "lsq", "lu_batched", "gemm"],
+                   default="all")
     p.add_argument("--count", type=int, default=65536)
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
@@ -366,6 +368,90 @@ def bench_lu(args):
     return out
 
 
+def bench_gemm(args):
+    """Matrix::gemm (C = A B, column-major DMatrix operands, entries uniform
+    [-1, 1]) through nrocm_dgemm / nrocm_sgemm: m = n = k = 512 (the size the
+    single-matrix parity case uses) and 8192.  A step is one product of
+    device-resident operands.  value = FP64 TFLOP/s at 8192^3; parity against
+    the oracle (nalgebra's gemv loop order) on the whole 512^3 product and on
+    sampled columns of the 8192^3 one, to 1e-12 (FP64) / 1e-5 (FP32) relative."""
+    import torch
+    import nalgebra_amd as nl
+    from nalgebra_amd import _lib
+    from oracle import oracle as orc
+    lib = _lib.lib()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+
+    def operands(n, seed, dtype):
+        a = np.zeros(n * n); b = np.zeros(n * n)
+        lib.nrocm_gen_uniform(seed, a.size, -1.0, 1.0, a.ctypes.data_as(C.c_void_p))
+        lib.nrocm_gen_uniform(seed + 1, b.size, -1.0, 1.0, b.ctypes.data_as(C.c_void_p))
+        A = nl.DMatrix(torch.from_numpy(a).to(dev, dtype), n, n)
+        B = nl.DMatrix(torch.from_numpy(b).to(dev, dtype), n, n)
+        return a, b, A, B
+
+    def timed(n, dtype, steps, warmup, seed):
+        a, b, A, B = operands(n, seed, dtype)
+        Cm = nl.DMatrix.zeros(n, n, dtype, dev)
+        for _ in range(warmup):
+            Cm.gemm(1.0, A, B, 0.0)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            Cm.gemm(1.0, A, B, 0.0)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        return ms, a, b, Cm
+
+    def rel(x, y):
+        return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+    res = {}
+    seed = args.seed + 300
+    for name, n, dtype, peak, tol in (("f64_512", 512, torch.float64, FP64_PEAK_TFLOPS, 1e-12),
+                                      ("f64_8192", 8192, torch.float64, FP64_PEAK_TFLOPS, 1e-12),
+                                      ("f32_8192", 8192, torch.float32, FP32_PEAK_TFLOPS, 1e-5)):
+        steps = max(1, args.steps) if n <= 512 else max(1, min(args.steps, 10))
+        ms, a, b, Cm = timed(n, dtype, steps * (20 if n <= 512 else 1), max(args.warmup, 1), seed)
+        tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+        npdt = np.float64 if dtype == torch.float64 else np.float32
+        Ah = a.reshape(n, n).T.astype(npdt); Bh = b.reshape(n, n).T.astype(npdt)
+        cols = np.arange(n) if n <= 512 else np.arange(0, n, n // 8)
+        ref = orc.gemm(npdt(1.0), Ah, np.asfortranarray(Bh[:, cols]))
+        got = Cm.numpy()[:, cols]
+        err = rel(got.astype(np.float64), ref.astype(np.float64))
+        if not err <= tol:
+            raise SystemExit(f"gemm {name}: relative error {err:.3e} vs the oracle exceeds {tol:g}")
+        res[name] = {"n": n, "ms": ms, "tflops": tf, "peak_tflops": peak, "frac": tf / peak,
+                     "parity": {"columns_checked": int(len(cols)), "rel_err_vs_oracle": err, "tol": tol}}
+    head = res["f64_8192"]
+    out = {"metric": METRIC, "value": head["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "gemm", "m": 8192, "n": 8192, "k": 8192, "layout": "column-major",
+                      "seed": seed},
+           "results": res,
+           "roofline": {"bound": "mfma", "achieved": head["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": head["frac"], "traffic": None,
+                        "note": "2 n^3 flops per product against the dense FP64 MFMA peak"}}
+    if not args.no_cpu:
+        a, b, _, _ = operands(512, seed, torch.float64)
+        Ah = a.reshape(512, 512).T.copy(); Bh = b.reshape(512, 512).T.copy()
+        t0 = time.perf_counter(); reps = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0) and reps < 200:
+            orc.gemm(1.0, Ah, Bh)
+            reps += 1
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": reps * 2.0 * 512 ** 3 / tc / 1e12, "unit": "TFLOP/s", "cores": 1,
+                               "kind": "port", **host_info(),
+                               "sample": f"{reps} oracle gemm calls at 512^3 (nalgebra's gemv loop order, one "
+                                         f"thread), {tc:.1f} s"}
+    return out
+
+
 def bench_lu_batched(args):
     """Batched uneven general systems: 65,536 f64 matrices, n_j uniform in
     [4, 64], entries uniform[-1, 1], 4 right-hand sides; one step = LU::new +
@@ -625,16 +711,19 @@ def main():
         out["cholesky_adjoint"] = bench_adjoint(sub)
         out["least_squares"] = bench_lsq(sub)
         out["lu_batched"] = bench_lu_batched(sub)
+        out["gemm"] = bench_gemm(sub)
         fast = argparse.Namespace(**vars(args))
         fast.mode, fast.no_cpu = "fast", True
         out["cholesky_fast"] = bench_chol(fast)
-        for k in ("lu_mixed_ir", "cholesky_adjoint", "least_squares", "lu_batched", "cholesky_fast"):
+        for k in ("lu_mixed_ir", "cholesky_adjoint", "least_squares", "lu_batched", "gemm", "cholesky_fast"):
             for f in ("metric", "n_gpus", "scaling", "vs_baseline", "data"):
                 out[k].pop(f, None)
     elif args.workload == "chol":
         out = bench_chol(args)
     elif args.workload == "lu_batched":
         out = bench_lu_batched(args)
+    elif args.workload == "gemm":
+        out = bench_gemm(args)
     elif args.workload == "lu":
         out = bench_lu(args)
     elif args.workload == "adjoint":
