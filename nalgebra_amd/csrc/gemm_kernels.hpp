@@ -170,8 +170,18 @@ template <typename TS> struct Vec16;
 template <> struct Vec16<double> { typedef double T __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
 template <> struct Vec16<float> { typedef float T __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
 
+// wavefronts per SIMD the register allocation is held to: two for the FP64
+// 128 x 128 shape (its 64 accumulator doubles per lane fit beside the operand
+// registers in 256 VGPRs, and the second wavefront hides the slice loads and
+// the barrier: 44 -> 67 TFLOP/s at 8192^3), the compiler's choice otherwise
+template <typename TS, typename SH>
+constexpr int gemm_waves()
+{
+    return sizeof(TS) == 8 && SH::BM == 128 ? 2 : 1;
+}
+
 template <typename TS, int MODE, typename SH>
-__global__ __launch_bounds__(256) void gemm_cm_kernel(int m, int n, int k, TS alpha, const TS *__restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gemm_waves<TS, SH>()))) void gemm_cm_kernel(int m, int n, int k, TS alpha, const TS *__restrict__ A,
                                                       long lda, const TS *__restrict__ B, long ldb, TS beta,
                                                       TS *__restrict__ C, long ldc, int vec_ok)
 {
