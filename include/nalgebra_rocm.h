@@ -121,7 +121,10 @@ int nrocm_dgetrs_noalloc(nrocm_handle handle, int n, int nrhs, const double *lu,
  * info[0] = 0, or j+1 for the first column whose pivot is not > 0 (nalgebra
  * returns None).  Shape of CholeskyScalar::xpotrf (nalgebra-lapack/src/cholesky.rs:181-197). */
 int nrocm_dpotrf(nrocm_handle handle, int n, double *a, int lda, int *info, int mode);
-/* Cholesky::solve_mut (cholesky.rs:122-129): B <- (L L^T)^{-1} B. */
+/* Cholesky::solve_mut (cholesky.rs:122-129): B <- (L L^T)^{-1} B, any n.
+ * n >= 1024: the forward sweep runs one 64-row block per workgroup; the
+ * EXACT transposed solve keeps nalgebra's dot order, one dependent chain of
+ * about n^2/16 additions on one wavefront. */
 int nrocm_dpotrs(nrocm_handle handle, int n, int nrhs, const double *l, int lda,
                  double *b, int ldb, int mode);
 

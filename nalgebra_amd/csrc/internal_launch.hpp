@@ -43,3 +43,7 @@ constexpr int NROCM_GETRS_LOOKBACK_MIN = 1024;  // nrocm_dgetrs uses it from thi
 size_t nrocm_getrs_scratch_bytes(int n);
 int nrocm_getrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const double *lu, int lda, const int *ipiv,
                          double *b, int ldb, int *info, bool exact, void *scratch);
+// Cholesky::solve_mut for large n (nrocm_dpotrs from NROCM_GETRS_LOOKBACK_MIN on),
+// same scratch size.
+int nrocm_potrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const double *L, int lda, double *b,
+                         int ldb, bool exact, void *scratch);

@@ -539,8 +539,12 @@ int nrocm_dpotrs(nrocm_handle h, int n, int nrhs, const double *l, int lda, doub
     if (ldb < std::max(1, n)) return -7;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -8;
     if (n == 0 || nrhs == 0) return NROCM_OK;
-    if ((size_t)n * sizeof(double) > 60 * 1024)
-        return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "potrs: n > 7680");
+    if (n >= NROCM_GETRS_LOOKBACK_MIN) {
+        void *ws = nullptr;
+        const int s = nrocm_workspace(h, nrocm_getrs_scratch_bytes(n), &ws);
+        if (s != NROCM_OK) return s;
+        return nrocm_potrs_lookback(h, h->stream, n, nrhs, l, lda, b, ldb, mode == NROCM_EXACT, ws);
+    }
     hipLaunchKernelGGL(potrs_kernel, dim3(1), dim3(256), (size_t)n * sizeof(double), h->stream, l, lda, n, b, ldb,
                        nrhs, mode == NROCM_EXACT ? 1 : 0);
     NROCM_CHECK_LAUNCH(h);

@@ -190,6 +190,155 @@ __global__ __launch_bounds__(LB) void lu_backward_lookback_kernel(const double *
     }
 }
 
+// ---------------------------------------------------------------- Cholesky::solve_mut
+// Cholesky::solve_mut (src/linalg/cholesky.rs:122-129): L y = b by the column
+// sweep of solve_lower_triangular_unchecked_mut (solve.rs:500-519,
+// x_i = b_i / L_ii), then L^T x = y by ad_solve_lower_triangular_unchecked_mut
+// (solve.rs:697-755), whose row i is x_i = (y_i - dot(L[i+1.., i], x[i+1..])) / L_ii.
+
+// Forward sweep, block k (ticket order) = rows [64k, 64k + 64); y is
+// published with `epoch` and written over b.
+template <bool EXACT>
+__global__ __launch_bounds__(LB) void chol_forward_lookback_kernel(const double *__restrict__ L, int lda, int n,
+                                                                   double *__restrict__ b, int *ticket,
+                                                                   unsigned long long *tags, unsigned epoch,
+                                                                   int *info)
+{
+    __shared__ double xs[LB];
+    __shared__ int my;
+    const int lane = threadIdx.x;
+    if (lane == 0) my = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int R = my;
+    const int r0 = R * LB;
+    const int rows = min(LB, n - r0);
+    const bool row_ok = lane < rows;
+    const int r = r0 + lane;
+    const double *Mrow = L + (row_ok ? r : r0);
+    double acc = row_ok ? b[r] : 0.0;
+    for (int J = 0; J < R; ++J) {
+        const int c0 = J * LB;
+        xs[lane] = await(tags, c0 + lane, epoch, info);
+        __syncthreads();
+        acc = apply_block<EXACT, true>(acc, Mrow, lda, c0, LB, xs, row_ok);
+        __syncthreads();
+    }
+    for (int i = 0; i < rows; ++i) {
+        if (lane == i) acc = acc / Mrow[(size_t)(r0 + i) * lda];
+        const double xi = lane_value(acc, i);
+        const double l = (lane > i && row_ok) ? Mrow[(size_t)(r0 + i) * lda] : 0.0;
+        if (lane > i) acc = sub_prod<EXACT>(acc, xi, l);
+    }
+    if (row_ok) {
+        publish(tags, r, epoch, acc);
+        b[r] = acc;
+    }
+}
+
+// NROCM_FAST transposed solve as a backward column sweep over U = L^T (any
+// summation order is allowed): block k (ticket order) = rows
+// [64(N-1-k), ...); lane r reads column r of L, U[r][c] = L[c + r lda].
+__global__ __launch_bounds__(LB) void chol_backward_lookback_fast_kernel(const double *__restrict__ L, int lda, int n,
+                                                                         double *__restrict__ b, int *ticket,
+                                                                         unsigned long long *tags, unsigned epoch_in,
+                                                                         unsigned epoch_out, int *info)
+{
+    __shared__ double xs[LB];
+    __shared__ int my;
+    const int lane = threadIdx.x;
+    const int nblk = (n + LB - 1) / LB;
+    if (lane == 0) my = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int k = my;
+    const int R = nblk - 1 - k;
+    const int r0 = R * LB;
+    const int rows = min(LB, n - r0);
+    const bool row_ok = lane < rows;
+    const int r = r0 + lane;
+    const double *Mcol = L + (size_t)(row_ok ? r : r0) * lda;
+    double acc = row_ok ? await(tags, r, epoch_in, info) : 0.0;
+    for (int j = 0; j < k; ++j) {
+        const int c0 = (nblk - 1 - j) * LB;
+        const int cols = min(LB, n - c0);
+        xs[lane] = lane < cols ? await(tags, c0 + lane, epoch_out, info) : 0.0;
+        __syncthreads();
+        double cur[PCH], nxt[PCH];
+#pragma unroll
+        for (int t = 0; t < PCH; ++t) cur[t] = (row_ok && t < cols) ? Mcol[c0 + t] : 0.0;
+        for (int s0 = 0; s0 < cols; s0 += PCH) {
+#pragma unroll
+            for (int t = 0; t < PCH; ++t) {
+                const int s = s0 + PCH + t;
+                nxt[t] = (row_ok && s < cols) ? Mcol[c0 + s] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < PCH; ++t)
+                if (s0 + t < cols) acc = fma(-xs[s0 + t], cur[t], acc);
+#pragma unroll
+            for (int t = 0; t < PCH; ++t) cur[t] = nxt[t];
+        }
+        __syncthreads();
+    }
+    for (int i = rows - 1; i >= 0; --i) {
+        if (lane == i) acc = acc / Mcol[r0 + i];
+        const double xi = lane_value(acc, i);
+        const double u = (lane < i) ? Mcol[r0 + i] : 0.0;
+        if (lane < i) acc = fma(-xi, u, acc);
+    }
+    if (row_ok) {
+        publish(tags, r, epoch_out, acc);
+        b[r] = acc;
+    }
+}
+
+// NROCM_EXACT transposed solve in nalgebra's dot order (the 8-accumulator dot
+// of src/base/blas.rs: accumulator q sums the products of elements q, q + 8,
+// ... of the full chunks in order, the sum is
+// ((((0 + (a0 + a4)) + (a1 + a5)) + (a2 + a6)) + (a3 + a7)) and the remaining
+// elements are added one by one).  Row i's first product needs x_{i+1}, so
+// the rows form one dependent chain: one wavefront, lanes 0..7 the
+// accumulators, x kept in LDS (XS_LDS) or, beyond the LDS, in b itself (each
+// new x_i is then released to the L2 and the vector L1 dropped before the
+// next row reads it: other lanes read values lane 0 wrote in this launch).
+template <bool XS_LDS>
+__global__ __launch_bounds__(64) void chol_backward_dot_kernel(const double *__restrict__ L, int lda, int n,
+                                                               double *b)
+{
+    extern __shared__ double xs_dyn[];
+    const int lane = threadIdx.x;
+    double *xb = XS_LDS ? xs_dyn : b;
+    if constexpr (XS_LDS) {
+        for (int r = lane; r < n; r += 64) xs_dyn[r] = b[r];
+        __syncthreads();
+    }
+    auto xload = [&](int idx) -> double { return xb[idx]; };
+    for (int i = n - 1; i >= 0; --i) {
+        const int len = n - i - 1;
+        const int full = len & ~7;
+        const double *li = L + (i + 1) + (size_t)i * lda;
+        double a = 0.0;
+        if (lane < 8) {
+#pragma unroll 4
+            for (int e = lane; e < full; e += 8) a += li[e] * xload(i + 1 + e);
+        }
+        double res = 0.0;
+        res += lane_value(a, 0) + lane_value(a, 4);
+        res += lane_value(a, 1) + lane_value(a, 5);
+        res += lane_value(a, 2) + lane_value(a, 6);
+        res += lane_value(a, 3) + lane_value(a, 7);
+        for (int e = full; e < len; ++e) res += li[e] * xload(i + 1 + e);
+        const double xi = (xload(i) - res) / L[i + (size_t)i * lda];
+        __syncthreads();  // every lane has read x_i's old value
+        if (lane == 0) xb[i] = xi;
+        if constexpr (!XS_LDS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if constexpr (!XS_LDS) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if constexpr (XS_LDS) {
+        for (int r = lane; r < n; r += 64) b[r] = xs_dyn[r];
+    }
+}
+
 // info = 1 + the largest i with U_ii == 0 (the backward sweep meets it first
 // and solve_mut returns false), or 0.
 __global__ void zero_diag_kernel(const double *__restrict__ LU, int lda, int n, int *info)
@@ -286,6 +435,62 @@ int nrocm_getrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const 
         else
             hipLaunchKernelGGL(lu_backward_lookback_kernel<false>, dim3(nblk), dim3(LB), 0, st, lu, lda, n, bc,
                                ticket, tags, e1, e2, info);
+        NROCM_CHECK_LAUNCH(h);
+    }
+    return NROCM_OK;
+}
+
+// Largest n whose x the exact transposed solve keeps in LDS (beyond it, in b).
+constexpr int CHOL_DOT_LDS_MAX = 19968;
+
+// Enqueues Cholesky::solve_mut on st with `scratch` (at least
+// nrocm_getrs_scratch_bytes(n), device memory): the forward sweep over one
+// 64-row block per workgroup, then the transposed solve (EXACT: one
+// wavefront in dot order; FAST: the backward look-back sweep over L^T).
+int nrocm_potrs_lookback(nrocm_handle h, hipStream_t st, int n, int nrhs, const double *L, int lda, double *b,
+                         int ldb, bool exact, void *scratch)
+{
+    char *base = (char *)scratch;
+    int *ticket = (int *)base;
+    int *timeout = ticket + 1;  // set by a look-back wait that ran out (never expected)
+    unsigned long long *tags =
+        (unsigned long long *)(base + 256 + (((size_t)n * sizeof(int) + 255) & ~(size_t)255));
+    const size_t ntag = 2 * (size_t)n;
+    hipLaunchKernelGGL(fill_words_kernel, dim3(nrocm_cdiv(std::max<size_t>(ntag, 1), 256)), dim3(256), 0, st, tags,
+                       ntag, timeout, 1);
+    NROCM_CHECK_LAUNCH(h);
+    const unsigned nblk = nrocm_cdiv(n, LB);
+    const bool lds = n <= CHOL_DOT_LDS_MAX;
+    if (exact && lds) {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)chol_backward_dot_kernel<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      CHOL_DOT_LDS_MAX * (int)sizeof(double));
+        });
+    }
+    unsigned epoch = 0;
+    for (int c = 0; c < nrhs; ++c) {
+        double *bc = b + (size_t)c * ldb;
+        const unsigned e1 = ++epoch, e2 = ++epoch;
+        hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, st, ticket);
+        if (exact)
+            hipLaunchKernelGGL(chol_forward_lookback_kernel<true>, dim3(nblk), dim3(LB), 0, st, L, lda, n, bc, ticket,
+                               tags, e1, timeout);
+        else
+            hipLaunchKernelGGL(chol_forward_lookback_kernel<false>, dim3(nblk), dim3(LB), 0, st, L, lda, n, bc,
+                               ticket, tags, e1, timeout);
+        if (exact) {
+            if (lds)
+                hipLaunchKernelGGL(chol_backward_dot_kernel<true>, dim3(1), dim3(64), (size_t)n * sizeof(double), st,
+                                   L, lda, n, bc);
+            else
+                hipLaunchKernelGGL(chol_backward_dot_kernel<false>, dim3(1), dim3(64), 0, st, L, lda, n, bc);
+        } else {
+            hipLaunchKernelGGL(reset_ticket_kernel, dim3(1), dim3(1), 0, st, ticket);
+            hipLaunchKernelGGL(chol_backward_lookback_fast_kernel, dim3(nblk), dim3(LB), 0, st, L, lda, n, bc, ticket,
+                               tags, e1, e2, timeout);
+        }
         NROCM_CHECK_LAUNCH(h);
     }
     return NROCM_OK;
