@@ -121,6 +121,19 @@ int nrocm_dgetrs_noalloc(nrocm_handle handle, int n, int nrhs, const double *lu,
  * info[0] = 0, or j+1 for the first column whose pivot is not > 0 (nalgebra
  * returns None).  Shape of CholeskyScalar::xpotrf (nalgebra-lapack/src/cholesky.rs:181-197). */
 int nrocm_dpotrf(nrocm_handle handle, int n, double *a, int lda, int *info, int mode);
+/* Cholesky::new_with_substitute (cholesky.rs:216-218, 237-264): a pivot that
+ * is not > 0 (NaN included) is replaced by `substitute` before its square
+ * root when substitute > 0; substitute <= 0 or NaN substitutes nothing (as
+ * Some(s) with no positive root does there), so 0 is Cholesky::new. */
+int nrocm_dpotrf_substitute(nrocm_handle handle, int n, double *a, int lda, double substitute,
+                            int *info, int mode);
+/* Determinant pieces from a factor's diagonal, written to the device array
+ * out[3]: out[0] = 1 * a_00 * a_11 * ... in index order (LU::determinant's
+ * product, lu.rs:301-314; the caller multiplies by the permutation's sign),
+ * out[1] = out[0]^2 (Cholesky::determinant, cholesky.rs:155-164), out[2] =
+ * sum of ln(a_ii^2) in index order (Cholesky::ln_determinant,
+ * cholesky.rs:166-184; ln is the device's, within 1 ulp of libm's). */
+int nrocm_ddiag_det(nrocm_handle handle, int n, const double *a, int lda, double *out);
 /* Cholesky::solve_mut (cholesky.rs:122-129): B <- (L L^T)^{-1} B, any n.
  * n >= 1024: the forward sweep runs one 64-row block per workgroup; the
  * EXACT transposed solve keeps nalgebra's dot order, one dependent chain of
@@ -216,6 +229,16 @@ int nrocm_sgeqrf(nrocm_handle handle, int m, int n, float *a, int lda, float *di
 /* QR::q_tr_mul (qr.rs:157-171): B (m x nrhs) <- Q^T B. */
 int nrocm_dqr_qtmul(nrocm_handle handle, int m, int n, const double *qr, int lda,
                     const double *diag, int nrhs, double *b, int ldb);
+/* Q B with QR::q's Q (qr.rs:108-129; the role of nalgebra-lapack's
+ * QrDecomposition::q_mul_mut, nalgebra-lapack/src/qr/abstraction.rs:83-90):
+ * B (m x nrhs) <- Q B, reflectors in reverse order, exact nalgebra arithmetic. */
+int nrocm_dqr_qmul(nrocm_handle handle, int m, int n, const double *qr, int lda,
+                   const double *diag, int nrhs, double *b, int ldb);
+/* R B with R the n x n upper triangle of QR::r (qr.rs:81-89), m >= n (the
+ * role of QrDecomposition::r_mul_mut, abstraction.rs:136-146): B (n x nrhs)
+ * <- R B in the reference BLAS TRMM order.  Uses the handle's workspace. */
+int nrocm_dqr_rmul(nrocm_handle handle, int m, int n, const double *qr, int lda,
+                   const double *diag, int nrhs, double *b, int ldb);
 /* QR::solve_mut's upper solve (qr.rs:224-256) on the leading n x n block:
  * B (n x nrhs) <- R^{-1} B.  info[0] = 0 or 1 + index of a zero diag. */
 int nrocm_dqr_rsolve(nrocm_handle handle, int n, const double *qr, int lda,

@@ -51,6 +51,8 @@ _PROTOS = {
     "nrocm_dgetrs_noalloc": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i]),
     "nrocm_dpotrf": (_i, [_vp, _i, _vp, _i, _vp, _i]),
     "nrocm_dpotrs": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _i]),
+    "nrocm_dpotrf_substitute": (_i, [_vp, _i, _vp, _i, _d, _vp, _i]),
+    "nrocm_ddiag_det": (_i, [_vp, _i, _vp, _i, _vp]),
     "nrocm_batch_create": (_i, [_vp, _i, _ip, _i, C.POINTER(_vp)]),
     "nrocm_batch_destroy": (_i, [_vp]),
     "nrocm_batch_set_graph": (_i, [_vp, _i]),
@@ -65,6 +67,8 @@ _PROTOS = {
     "nrocm_sgeqrf": (_i, [_vp, _i, _i, _vp, _i, _vp]),
     "nrocm_dqr_qtmul": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i]),
     "nrocm_dqr_rsolve": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp]),
+    "nrocm_dqr_qmul": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i]),
+    "nrocm_dqr_rmul": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i]),
     "nrocm_sgels_batched": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _vp]),
     "nrocm_dgesv_ir": (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _d, _i, _ip, _dp]),
     "nrocm_ir_fallback": (_i, [_vp, _ip]),

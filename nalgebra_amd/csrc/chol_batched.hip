@@ -25,7 +25,7 @@ namespace {
 // ------------------------------------------------------------------ dispatch tables
 
 using FacLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
-                         double *const *, int *, double *, int, int);
+                         double *const *, int *, double *, int, int, double);
 using SolLaunch = int (*)(bool, unsigned, hipStream_t, const int *, int, int, const int *,
                          double *const *, double *const *, int, const int *, double *, double *,
                          int, int, int);
@@ -242,10 +242,10 @@ void report_lanes(nrocm_batch bt, int mi, int *lanes)
 // ------------------------------------------------------------------ internal entry for single matrices
 
 int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int ldb, int nrhs,
-                      int *info, bool exact)
+                      int *info, bool exact, double substitute)
 {
     const int bin = std::max(0, (n + 7) / 8 - 1);
-    if (kFac[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, info, a, n, lda))
+    if (kFac[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, info, a, n, lda, substitute))
         NROCM_CHECK_LAUNCH(h);
     if (b && nrhs > 0 &&
         kSol[bin](exact, 1, h->stream, nullptr, 0, 1, nullptr, nullptr, nullptr, nrhs, info, a, b, n, lda, ldb))

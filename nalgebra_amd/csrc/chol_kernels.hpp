@@ -92,9 +92,12 @@ __device__ __forceinline__ double lane_value(double v, int src)
 // The failing column is found afterwards: pivot k failed iff its square root
 // is not > 0 (d <= 0 gives NaN or +-0, NaN gives NaN), and nalgebra stops at
 // the first such k.  mydiag receives L_rr (the square root taken at step r).
+// sub > 0 is Cholesky::new_with_substitute's substitute (cholesky.rs:216-218,
+// 237-264): a pivot that is not > 0 (NaN included) is replaced by it before
+// the square root; sub <= 0 or NaN substitutes nothing, as there.
 template <int NB, int G, bool EXACT>
 __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch, int n, int g, int r, int lane,
-                                      double &mydiag)
+                                      double &mydiag, double sub = 0.0)
 {
     mydiag = 1.0;
 #pragma unroll
@@ -107,7 +110,7 @@ __device__ __forceinline__ int factor(double (&a)[NB], double *col, double *xch,
             lds_barrier();
             dkk = col[g * G + k];
         }
-        const double den = sqrt(dkk);
+        const double den = sqrt((sub > 0.0 && !(dkk > 0.0)) ? sub : dkk);
         if (r == k) {
             a[k] = den;
             mydiag = den;
@@ -612,7 +615,7 @@ __device__ __forceinline__ Member member(const int *order, int begin, int end, c
 template <int NB, int G, bool EXACT>
 __global__ __launch_bounds__(64) void potrf_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz,
-    double *const *__restrict__ Aptr, int *__restrict__ info, double *A1, int n1, int lda1)
+    double *const *__restrict__ Aptr, int *__restrict__ info, double *A1, int n1, int lda1, double sub)
 {
     __shared__ Smem<64 / G> s;
     const int lane = threadIdx.x;
@@ -624,7 +627,7 @@ __global__ __launch_bounds__(64) void potrf_kernel(
     load_lower<NB>(a, mb.A, mb.lda, mb.n, r);
     (void)nmax;
     double mydiag;
-    const int fail = factor<NB, G, EXACT>(a, s.col, s.xch[0], mb.n, g, r, lane, mydiag);
+    const int fail = factor<NB, G, EXACT>(a, s.col, s.xch[0], mb.n, g, r, lane, mydiag, sub);
     store_lower<NB>(a, mb.A, mb.lda, mb.n, r);
     if (mb.valid && r == 0 && info) info[order ? mb.jm : 0] = fail;
 }
@@ -823,7 +826,7 @@ int fused_dispatch(bool exact, int G, hipStream_t st, const int *order, int begi
 #define NROCM_CHOL_DECLARE(NB)                                                                    \
     int nrocm_chol_potrf_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
                                      int begin, int end, const int *nsz, double *const *A,        \
-                                     int *info, double *A1, int n1, int lda1);                    \
+                                     int *info, double *A1, int n1, int lda1, double sub);        \
     int nrocm_chol_potrs_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
                                      int begin, int end, const int *nsz, double *const *A,        \
                                      double *const *B, int nrhs, const int *info, double *A1,     \
@@ -839,16 +842,16 @@ int fused_dispatch(bool exact, int G, hipStream_t st, const int *order, int begi
 #define NROCM_CHOL_DEFINE(NB)                                                                     \
     int nrocm_chol_potrf_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \
                                      int begin, int end, const int *nsz, double *const *A,        \
-                                     int *info, double *A1, int n1, int lda1)                     \
+                                     int *info, double *A1, int n1, int lda1, double sub)         \
     {                                                                                             \
         using namespace nrocm_chol;                                                               \
         constexpr int G = group_for<NB>();                                                        \
         if (exact)                                                                                \
             hipLaunchKernelGGL((potrf_kernel<NB, G, true>), dim3(grid), dim3(64), 0, st, order,   \
-                               begin, end, nsz, A, info, A1, n1, lda1);                           \
+                               begin, end, nsz, A, info, A1, n1, lda1, sub);                      \
         else                                                                                      \
             hipLaunchKernelGGL((potrf_kernel<NB, G, false>), dim3(grid), dim3(64), 0, st, order,  \
-                               begin, end, nsz, A, info, A1, n1, lda1);                           \
+                               begin, end, nsz, A, info, A1, n1, lda1, sub);                      \
         return hipGetLastError() == hipSuccess ? 0 : 1;                                           \
     }                                                                                             \
     int nrocm_chol_potrs_launch_##NB(bool exact, unsigned grid, hipStream_t st, const int *order, \

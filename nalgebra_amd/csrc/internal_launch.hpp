@@ -34,8 +34,9 @@ int nrocm_getrf_window_s(nrocm_handle h, int n, float *A, int lda, int *ipiv, in
                          NrocmTrailing trail);
 
 // chol_batched.hip: n <= 64 single-matrix path.
+// substitute > 0: Cholesky::new_with_substitute's substitute for pivots that are not > 0.
 int nrocm_potrf_small(nrocm_handle h, int n, double *a, int lda, double *b, int ldb, int nrhs, int *info,
-                      bool exact);
+                      bool exact, double substitute = 0.0);
 
 // solve_lookback.hip: LU::solve_mut for any n, one 64-row block per workgroup.
 // Scratch: nrocm_getrs_scratch_bytes(n) bytes of device memory; allocates nothing.

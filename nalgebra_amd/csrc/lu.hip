@@ -256,7 +256,7 @@ __global__ void chol_info_merge_kernel(int *info, const int *blk, int j0)
     if (*info == 0 && *blk != 0) *info = j0 + *blk;
 }
 
-int potrf_blocked(nrocm_handle h, int n, double *A, int lda, int *info, bool exact)
+int potrf_blocked(nrocm_handle h, int n, double *A, int lda, int *info, bool exact, double substitute)
 {
     hipStream_t st = h->stream;
     int *blk = (int *)h->small;  // per-block info scratch
@@ -265,7 +265,7 @@ int potrf_blocked(nrocm_handle h, int n, double *A, int lda, int *info, bool exa
     for (int j0 = 0; j0 < n; j0 += 64) {
         const int nb = std::min(64, n - j0);
         double *D = A + j0 + (size_t)j0 * lda;
-        int st2 = nrocm_potrf_small(h, nb, D, lda, nullptr, 0, 0, blk, exact);
+        int st2 = nrocm_potrf_small(h, nb, D, lda, nullptr, 0, 0, blk, exact, substitute);
         if (st2 != NROCM_OK) return st2;
         hipLaunchKernelGGL(chol_info_merge_kernel, dim3(1), dim3(1), 0, st, info, blk, j0);
         LU_TRY(hipGetLastError());
@@ -284,6 +284,36 @@ int potrf_blocked(nrocm_handle h, int n, double *A, int lda, int *info, bool exa
         }
     }
     return NROCM_OK;
+}
+
+// out[0] = the product of the diagonal, 1 * a_00 * a_11 * ... in index order
+// (LU::determinant's product, lu.rs:301-314, and Cholesky::determinant's,
+// cholesky.rs:155-164), out[1] = out[0] * out[0] (Cholesky::determinant's
+// modulus squared), out[2] = 0 + ln(a_00^2) + ln(a_11^2) + ... in index order
+// (Cholesky::ln_determinant, cholesky.rs:166-184).  Products and sums are
+// one dependent chain: lane 0 walks it while the others prefetch the diagonal.
+__global__ __launch_bounds__(64) void diag_det_kernel(const double *A, int lda, int n, double *out)
+{
+    __shared__ double d[64];
+    const int lane = threadIdx.x;
+    double prod = 1.0, lnsum = 0.0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        d[lane] = (i0 + lane < n) ? A[(size_t)(i0 + lane) * ((size_t)lda + 1)] : 1.0;
+        __syncthreads();
+        if (lane == 0) {
+            const int cnt = min(64, n - i0);
+            for (int q = 0; q < cnt; ++q) {
+                prod = prod * d[q];
+                lnsum = lnsum + log(d[q] * d[q]);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        out[0] = prod;
+        out[1] = prod * prod;
+        out[2] = lnsum;
+    }
 }
 
 // Cholesky solve for n > 64 on one workgroup: forward (solve.rs:488-519) and
@@ -525,7 +555,31 @@ int nrocm_dpotrf(nrocm_handle h, int n, double *a, int lda, int *info, int mode)
     if (!info) return -5;
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -6;
     if (n <= 64) return nrocm_potrf_small(h, n, a, lda, nullptr, 0, 0, info, mode == NROCM_EXACT);
-    return potrf_blocked(h, n, a, lda, info, mode == NROCM_EXACT);
+    return potrf_blocked(h, n, a, lda, info, mode == NROCM_EXACT, 0.0);
+}
+
+int nrocm_dpotrf_substitute(nrocm_handle h, int n, double *a, int lda, double substitute, int *info, int mode)
+{
+    if (!h) return -1;
+    if (n < 0) return -2;
+    if (!a && n > 0) return -3;
+    if (lda < std::max(1, n)) return -4;
+    if (!info) return -6;
+    if (mode != NROCM_EXACT && mode != NROCM_FAST) return -7;
+    if (n <= 64) return nrocm_potrf_small(h, n, a, lda, nullptr, 0, 0, info, mode == NROCM_EXACT, substitute);
+    return potrf_blocked(h, n, a, lda, info, mode == NROCM_EXACT, substitute);
+}
+
+int nrocm_ddiag_det(nrocm_handle h, int n, const double *a, int lda, double *out)
+{
+    if (!h) return -1;
+    if (n < 0) return -2;
+    if (!a && n > 0) return -3;
+    if (lda < std::max(1, n)) return -4;
+    if (!out) return -5;
+    hipLaunchKernelGGL(diag_det_kernel, dim3(1), dim3(64), 0, h->stream, a, lda, n, out);
+    NROCM_CHECK_LAUNCH(h);
+    return NROCM_OK;
 }
 
 int nrocm_dpotrs(nrocm_handle h, int n, int nrhs, const double *l, int lda, double *b, int ldb, int mode)

@@ -146,6 +146,50 @@ __global__ __launch_bounds__(256) void qtmul_exact_kernel(const double *QR, int 
     }
 }
 
+// B (m x nrhs) <- Q B, Q as QR::q builds it (qr.rs:108-129): the reflectors
+// in reverse order, each with the sign of its diagonal entry.  The product
+// with the identity reproduces QR::q (up to the sign of zero entries left of
+// the diagonal, which q() never touches).
+__global__ __launch_bounds__(256) void qmul_exact_kernel(const double *QR, int m, int n, int lda, const double *diag,
+                                                         double *B, int ldb, int nrhs)
+{
+    __shared__ double scratch[256];
+    const int mn = m < n ? m : n;
+    for (int i = mn - 1; i >= 0; --i) {
+        reflect_cols<double>(QR + i + (size_t)i * lda, m - i, B + i, ldb, 0, nrhs, signum_rs(diag[i]), scratch);
+        __syncthreads();
+    }
+}
+
+// B (n x nrhs) <- R B with R the n x n upper triangle of QR::r (qr.rs:81-89:
+// |diag| on the diagonal), in the reference BLAS TRMM order (side left,
+// upper, no transpose): row i is ((b_i r_ii + b_{i+1} r_{i,i+1}) + ...) over
+// the input values, a zero b_k contributing nothing.  One thread per row; the
+// input column is read from B0 (a copy), the result written to B.
+__global__ __launch_bounds__(256) void rmul_kernel(const double *QR, int n, int lda, const double *diag,
+                                                   const double *B0, double *B, int ldb, int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = blockIdx.y; c < nrhs; c += gridDim.y) {
+        if (i >= n) continue;
+        const double *b0 = B0 + (size_t)c * n;
+        const double bi = b0[i];
+        double acc = (bi != 0.0) ? bi * fabs(diag[i]) : bi;
+        const double *rrow = QR + i;
+        for (int k = i + 1; k < n; ++k) {
+            const double bk = b0[k];
+            if (bk != 0.0) acc = acc + bk * rrow[(size_t)k * lda];
+        }
+        B[i + (size_t)c * ldb] = acc;
+    }
+}
+
+__global__ void copy_cols_kernel(const double *B, int ldb, int n, int nrhs, double *B0)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (size_t)n * nrhs) B0[e] = B[(e % n) + (e / n) * (size_t)ldb];
+}
+
 // QR's upper solve (qr.rs:224-256) on the leading n x n block.
 __global__ __launch_bounds__(256) void rsolve_exact_kernel(const double *QR, int n, int lda, const double *diag,
                                                            double *B, int ldb, int nrhs, int *info)
@@ -238,6 +282,49 @@ int nrocm_dqr_qtmul(nrocm_handle h, int m, int n, const double *qr, int lda, con
     if (ldb < std::max(1, m)) return -9;
     if (std::min(m, n) == 0 || nrhs == 0) return NROCM_OK;
     hipLaunchKernelGGL(qtmul_exact_kernel, dim3(1), dim3(256), 0, h->stream, qr, m, n, lda, diag, b, ldb, nrhs);
+    NROCM_CHECK_LAUNCH(h);
+    return NROCM_OK;
+}
+
+int nrocm_dqr_qmul(nrocm_handle h, int m, int n, const double *qr, int lda, const double *diag, int nrhs, double *b,
+                   int ldb)
+{
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (!qr && m > 0 && n > 0) return -4;
+    if (lda < std::max(1, m)) return -5;
+    if (!diag && std::min(m, n) > 0) return -6;
+    if (nrhs < 0) return -7;
+    if (!b && m > 0 && nrhs > 0) return -8;
+    if (ldb < std::max(1, m)) return -9;
+    if (std::min(m, n) == 0 || nrhs == 0) return NROCM_OK;
+    hipLaunchKernelGGL(qmul_exact_kernel, dim3(1), dim3(256), 0, h->stream, qr, m, n, lda, diag, b, ldb, nrhs);
+    NROCM_CHECK_LAUNCH(h);
+    return NROCM_OK;
+}
+
+int nrocm_dqr_rmul(nrocm_handle h, int m, int n, const double *qr, int lda, const double *diag, int nrhs, double *b,
+                   int ldb)
+{
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0 || n > m) return -3;
+    if (!qr && n > 0) return -4;
+    if (lda < std::max(1, m)) return -5;
+    if (!diag && n > 0) return -6;
+    if (nrhs < 0) return -7;
+    if (!b && n > 0 && nrhs > 0) return -8;
+    if (ldb < std::max(1, n)) return -9;
+    if (n == 0 || nrhs == 0) return NROCM_OK;
+    void *ws = nullptr;
+    const int s = nrocm_workspace(h, (size_t)n * nrhs * sizeof(double), &ws);
+    if (s != NROCM_OK) return s;
+    double *b0 = (double *)ws;
+    const size_t tot = (size_t)n * nrhs;
+    hipLaunchKernelGGL(copy_cols_kernel, dim3(nrocm_cdiv(tot, 256)), dim3(256), 0, h->stream, b, ldb, n, nrhs, b0);
+    hipLaunchKernelGGL(rmul_kernel, dim3(nrocm_cdiv(n, 256), std::min(nrhs, 65535)), dim3(256), 0, h->stream, qr, n,
+                       lda, diag, b0, b, ldb, nrhs);
     NROCM_CHECK_LAUNCH(h);
     return NROCM_OK;
 }

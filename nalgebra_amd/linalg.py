@@ -215,16 +215,25 @@ class LU:
         """LU::determinant (lu.rs:301-314)."""
         if not self.lu.is_square():
             raise ValueError("LU determinant: non-square matrix.")
-        d = torch.diagonal(self.lu.data.view(self.lu.ncols, self.lu.nrows)).cpu().numpy()
-        res = 1.0
-        for v in d:
-            res = res * float(v)
-        return res * self.p.determinant()
+        return _diag_det(self.lu)[0] * self.p.determinant()
 
     def try_inverse(self) -> Optional[DMatrix]:
         """LU::try_inverse (lu.rs:266-297)."""
         out = DMatrix.identity(self.lu.nrows, device=self.lu.data.device)
         return out if self.solve_mut(out) else None
+
+
+def _diag_det(m: DMatrix):
+    """(prod of the diagonal, its square, sum of ln(d_ii^2)) of a square
+    factor, each in index order, computed on the device (nrocm_ddiag_det)."""
+    if m.dtype != torch.float64:
+        raise TypeError("determinant: float64 only")
+    out = torch.empty(3, dtype=torch.float64, device=m.data.device)
+    h = handle()
+    st = lib().nrocm_ddiag_det(h, m.nrows, _ptr(m.data), max(m.nrows, 1), _ptr(out))
+    check("nrocm_ddiag_det", st, h)
+    v = out.cpu().numpy()
+    return float(v[0]), float(v[1]), float(v[2])
 
 
 class Cholesky:
@@ -236,14 +245,22 @@ class Cholesky:
 
     @classmethod
     def new(cls, m: DMatrix, mode: int = EXACT) -> Optional["Cholesky"]:
+        """Cholesky::new (cholesky.rs:196-198), in place on m."""
+        return cls.new_with_substitute(m, 0.0, mode)
+
+    @classmethod
+    def new_with_substitute(cls, m: DMatrix, substitute: float, mode: int = EXACT) -> Optional["Cholesky"]:
+        """Cholesky::new_with_substitute (cholesky.rs:216-218): pivots that are
+        not > 0 take `substitute` when it is > 0.  In place on m."""
         if not m.is_square():
             raise ValueError("The input matrix must be square.")
         if m.dtype != torch.float64:
             raise TypeError("Cholesky: float64 only")
         info = torch.zeros(1, dtype=torch.int32, device=m.data.device)
         h = handle()
-        st = lib().nrocm_dpotrf(h, m.nrows, _ptr(m.data), max(m.nrows, 1), _ptr(info), mode)
-        check("nrocm_dpotrf", st, h)
+        st = lib().nrocm_dpotrf_substitute(h, m.nrows, _ptr(m.data), max(m.nrows, 1), float(substitute), _ptr(info),
+                                           mode)
+        check("nrocm_dpotrf_substitute", st, h)
         return None if int(info.item()) != 0 else cls(m, mode)
 
     def l_dirty(self) -> DMatrix:
@@ -265,6 +282,20 @@ class Cholesky:
         x = b.clone()
         self.solve_mut(x)
         return x
+
+    def inverse(self) -> DMatrix:
+        """Cholesky::inverse (cholesky.rs:146-153): solve_mut on the identity."""
+        out = DMatrix.identity(self.chol.nrows, device=self.chol.data.device)
+        self.solve_mut(out)
+        return out
+
+    def determinant(self) -> float:
+        """Cholesky::determinant (cholesky.rs:155-164): (prod L_ii)^2."""
+        return _diag_det(self.chol)[1]
+
+    def ln_determinant(self) -> float:
+        """Cholesky::ln_determinant (cholesky.rs:166-184): sum of ln(L_ii^2)."""
+        return _diag_det(self.chol)[2]
 
     def backward(self, x: DMatrix, dx: DMatrix):
         """Adjoint of x = A^{-1} b: returns (dA symmetrised, db) for upstream dx."""
@@ -302,6 +333,32 @@ class QR:
         st = lib().nrocm_dqr_qtmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), max(self.qr.nrows, 1),
                                    _ptr(self.diag), b.ncols, _ptr(b.data), max(b.nrows, 1))
         check("nrocm_dqr_qtmul", st, h)
+
+    def q_mul(self, b: DMatrix) -> None:
+        """b <- Q b with QR::q's Q (qr.rs:108-129), in place: the role of
+        nalgebra-lapack's q_mul_mut (qr/abstraction.rs:83-90)."""
+        if self.qr.dtype != torch.float64:
+            raise TypeError("q_mul: float64 only")
+        if b.nrows != self.qr.nrows:
+            raise ValueError("q_mul: b must have as many rows as the decomposed matrix")
+        h = handle()
+        st = lib().nrocm_dqr_qmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), max(self.qr.nrows, 1),
+                                  _ptr(self.diag), b.ncols, _ptr(b.data), max(b.nrows, 1))
+        check("nrocm_dqr_qmul", st, h)
+
+    def r_mul(self, b: DMatrix) -> None:
+        """b <- R b with R = QR::r()'s n x n upper triangle (m >= n), in place:
+        the role of nalgebra-lapack's r_mul_mut (qr/abstraction.rs:136-146)."""
+        if self.qr.dtype != torch.float64:
+            raise TypeError("r_mul: float64 only")
+        if self.qr.nrows < self.qr.ncols:
+            raise ValueError("r_mul: R is square only for nrows >= ncols")
+        if b.nrows != self.qr.ncols:
+            raise ValueError("r_mul: b must have as many rows as the decomposed matrix has columns")
+        h = handle()
+        st = lib().nrocm_dqr_rmul(h, self.qr.nrows, self.qr.ncols, _ptr(self.qr.data), max(self.qr.nrows, 1),
+                                  _ptr(self.diag), b.ncols, _ptr(b.data), max(b.nrows, 1))
+        check("nrocm_dqr_rmul", st, h)
 
     def solve_mut(self, b: DMatrix) -> bool:
         """QR::solve_mut (qr.rs:204-221): square systems only."""

@@ -320,6 +320,35 @@ void F(qr_qtmul)(long m, long n, const T *QR, long lda, const T *diag, long nrhs
         F(reflect_with_sign)(m - i, QR + i + i * lda, nrhs, B + i, ldb, F(signum)(diag[i]));
 }
 
+/* Q B with QR::q's Q (qr.rs:108-129: the reflectors in reverse order, each
+ * with the signum of its diag entry) applied to B (m x nrhs) instead of the
+ * identity. */
+void F(qr_qmul)(long m, long n, const T *QR, long lda, const T *diag, long nrhs, T *B, long ldb)
+{
+    long mn = m < n ? m : n;
+    for (long i = mn - 1; i >= 0; --i)
+        F(reflect_with_sign)(m - i, QR + i + i * lda, nrhs, B + i, ldb, F(signum)(diag[i]));
+}
+
+/* R B in place, R = the n x n upper triangle of QR::r (qr.rs:81-89, R_kk =
+ * |diag_k|), in the reference BLAS TRMM loop (side left, upper, no
+ * transpose, non-unit, alpha 1): for each column, for k ascending, a nonzero
+ * b_k adds b_k R_ik to every b_i above it and becomes b_k R_kk. */
+void F(qr_rmul)(long n, const T *QR, long lda, const T *diag, long nrhs, T *B, long ldb)
+{
+    for (long j = 0; j < nrhs; ++j) {
+        T *b = B + j * ldb;
+        for (long k = 0; k < n; ++k) {
+            if (b[k] != (T)0) {
+                T temp = b[k];
+                for (long i = 0; i < k; ++i) b[i] = b[i] + temp * QR[i + k * lda];
+                temp = temp * FABS(diag[k]);
+                b[k] = temp;
+            }
+        }
+    }
+}
+
 /* QR's private solve_upper_triangular_mut: qr.rs:224-256, on the leading
  * n x n block (R_ii = |diag_i|, R_ri = QR[r,i] above the diagonal).
  * Returns 0 on an exact-zero diagonal. */

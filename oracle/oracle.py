@@ -58,6 +58,8 @@ def lib():
             f = getattr(_lib, f"nro_{suf}_potrs"); f.restype = None; f.argtypes = [_L, _L, P, _L, P, _L]
             f = getattr(_lib, f"nro_{suf}_geqrf"); f.restype = None; f.argtypes = [_L, _L, P, _L, P]
             f = getattr(_lib, f"nro_{suf}_qr_qtmul"); f.restype = None; f.argtypes = [_L, _L, P, _L, P, _L, P, _L]
+            f = getattr(_lib, f"nro_{suf}_qr_qmul"); f.restype = None; f.argtypes = [_L, _L, P, _L, P, _L, P, _L]
+            f = getattr(_lib, f"nro_{suf}_qr_rmul"); f.restype = None; f.argtypes = [_L, P, _L, P, _L, P, _L]
             f = getattr(_lib, f"nro_{suf}_qr_rsolve"); f.restype = C.c_int; f.argtypes = [_L, P, _L, P, _L, P, _L]
             f = getattr(_lib, f"nro_{suf}_qr_ls"); f.restype = C.c_int; f.argtypes = [_L, _L, P, _L, P, P, P]
         _lib.nro_d_potrs_adjoint.restype = None
@@ -243,6 +245,45 @@ def qr_r(QR, diag):
     R = np.triu(np.array(QR[:k, :], copy=True))
     R[np.arange(k), np.arange(k)] = np.abs(diag)
     return R
+
+
+def qr_qmul(QR, diag, B):
+    """Q B with QR::q's Q (qr.rs:108-129), the reflectors applied in reverse."""
+    QR = _fmat(QR); X = _fmat(B, QR.dtype)
+    m, n = QR.shape
+    diag = np.ascontiguousarray(diag, dtype=QR.dtype)
+    _fn(QR.dtype, "qr_qmul")(m, n, _ptr(QR), m, _ptr(diag), X.shape[1], _ptr(X), m)
+    return X
+
+
+def qr_rmul(QR, diag, B):
+    """R B with R = QR::r()'s n x n upper triangle (m >= n), reference TRMM order."""
+    QR = _fmat(QR); X = _fmat(B, QR.dtype)
+    m, n = QR.shape
+    assert m >= n and X.shape[0] == n
+    diag = np.ascontiguousarray(diag, dtype=QR.dtype)
+    _fn(QR.dtype, "qr_rmul")(n, _ptr(QR), m, _ptr(diag), X.shape[1], _ptr(X), n)
+    return X
+
+
+def diag_product(M):
+    """1 * m_00 * m_11 * ... in index order: LU::determinant's product
+    (lu.rs:301-314) and Cholesky::determinant's (cholesky.rs:155-164, which
+    then squares it)."""
+    M = np.asarray(M)
+    res = M.dtype.type(1)
+    for i in range(min(M.shape)):
+        res = res * M[i, i]
+    return res
+
+
+def chol_ln_determinant(L):
+    """Cholesky::ln_determinant (cholesky.rs:166-184): 0 + ln(L_00^2) + ... in order."""
+    L = np.asarray(L, dtype=np.float64)
+    res = 0.0
+    for i in range(L.shape[0]):
+        res = res + float(np.log(L[i, i] * L[i, i]))
+    return res
 
 
 def qr_q(QR, diag):

@@ -73,6 +73,17 @@ def test_lu_cholesky_argument_errors(env):
     assert L.nrocm_dpotrs(h, 2, 1, d, 2, None, 2, EXACT) == -6
     assert L.nrocm_dpotrs(h, 2, 1, d, 2, d, 1, EXACT) == -7
     assert L.nrocm_dpotrs(h, 2, 1, d, 2, d, 2, 5) == -8
+    # nrocm_dpotrf_substitute(h, n, a, lda, substitute, info, mode)
+    assert L.nrocm_dpotrf_substitute(h, -1, d, 2, 1.0, i, EXACT) == -2
+    assert L.nrocm_dpotrf_substitute(h, 2, None, 2, 1.0, i, EXACT) == -3
+    assert L.nrocm_dpotrf_substitute(h, 2, d, 1, 1.0, i, EXACT) == -4
+    assert L.nrocm_dpotrf_substitute(h, 2, d, 2, 1.0, None, EXACT) == -6
+    assert L.nrocm_dpotrf_substitute(h, 2, d, 2, 1.0, i, 9) == -7
+    # nrocm_ddiag_det(h, n, a, lda, out)
+    assert L.nrocm_ddiag_det(h, -1, d, 2, d) == -2
+    assert L.nrocm_ddiag_det(h, 2, None, 2, d) == -3
+    assert L.nrocm_ddiag_det(h, 2, d, 1, d) == -4
+    assert L.nrocm_ddiag_det(h, 2, d, 2, None) == -5
 
 
 def test_qr_and_mixed_argument_errors(env):
@@ -87,6 +98,12 @@ def test_qr_and_mixed_argument_errors(env):
     assert L.nrocm_dqr_qtmul(h, 2, 2, d, 2, d, -1, d, 2) == -7
     assert L.nrocm_dqr_qtmul(h, 2, 2, d, 2, d, 1, None, 2) == -8
     assert L.nrocm_dqr_qtmul(h, 2, 2, d, 2, d, 1, d, 1) == -9
+    # nrocm_dqr_qmul / nrocm_dqr_rmul(h, m, n, qr, lda, diag, nrhs, b, ldb)
+    assert L.nrocm_dqr_qmul(h, 2, 2, d, 2, d, -1, d, 2) == -7
+    assert L.nrocm_dqr_qmul(h, 2, 2, d, 2, d, 1, d, 1) == -9
+    assert L.nrocm_dqr_rmul(h, 2, 3, d, 2, d, 1, d, 3) == -3     # R is square only for m >= n
+    assert L.nrocm_dqr_rmul(h, 3, 2, d, 2, d, 1, d, 2) == -5
+    assert L.nrocm_dqr_rmul(h, 3, 2, d, 3, d, 1, d, 1) == -9
     # nrocm_dqr_rsolve(h, n, qr, lda, diag, nrhs, b, ldb, info)
     assert L.nrocm_dqr_rsolve(h, 2, d, 1, d, 1, d, 2, i) == -4
     assert L.nrocm_dqr_rsolve(h, 2, d, 2, d, 1, d, 2, None) == -9

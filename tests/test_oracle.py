@@ -313,6 +313,60 @@ def test_qr_qtmul_matches_q(orc):
     assert np.allclose(QtB[:8], Q.T @ B, atol=1e-13)
 
 
+def test_qr_qmul_pins(orc):
+    """Q B: on the identity it rebuilds QR::q (value for value; q() leaves the
+    entries left of each reflector alone, so only zero signs may differ),
+    it inverts q_tr_mul, and Q [R; 0] reproduces A."""
+    rng = np.random.default_rng(22)
+    for shape in ((30, 8), (6, 6), (5, 9)):
+        m, n = shape
+        A = rng.uniform(-1, 1, shape)
+        QR, diag = orc.geqrf(A)
+        k = min(shape)
+        QI = orc.qr_qmul(QR, diag, np.eye(m))
+        assert np.array_equal(QI[:, :k], orc.qr_q(QR, diag))
+        B = rng.uniform(-1, 1, (m, 3))
+        assert np.allclose(orc.qr_qmul(QR, diag, orc.qr_qtmul(QR, diag, B)), B, atol=1e-14)
+        R = orc.qr_r(QR, diag)
+        Rpad = np.zeros((m, n)); Rpad[:k] = R
+        assert np.allclose(orc.qr_qmul(QR, diag, Rpad), A, atol=1e-14)
+
+
+def test_qr_rmul_pins(orc):
+    """R B in TRMM order: exact on small integers (any order is), the skipped
+    zero keeps its sign, and real inputs agree with numpy's product."""
+    rng = np.random.default_rng(23)
+    n = 9
+    QR = rng.integers(-5, 6, (12, n)).astype(np.float64)
+    diag = rng.integers(-5, 6, n).astype(np.float64)
+    B = rng.integers(-5, 6, (n, 4)).astype(np.float64)
+    R = np.triu(QR[:n]); R[np.arange(n), np.arange(n)] = np.abs(diag)
+    assert np.array_equal(orc.qr_rmul(QR, diag, B), R @ B)
+    # a zero b_k contributes nothing: -0.0 stays -0.0
+    got = orc.qr_rmul(np.array([[7.0, 3.0], [1.0, 7.0], [0.0, 0.0]]), np.array([2.0, -5.0]),
+                      np.array([[-0.0], [0.0]]))
+    assert np.signbit(got[0, 0]) and not np.signbit(got[1, 0])
+    A = rng.uniform(-1, 1, (40, 12))
+    QR, diag = orc.geqrf(A)
+    B = rng.uniform(-1, 1, (12, 3))
+    ref = orc.qr_r(QR, diag) @ B
+    assert np.linalg.norm(orc.qr_rmul(QR, diag, B) - ref) / np.linalg.norm(ref) < 1e-15
+
+
+def test_cholesky_determinants_closed_form(orc):
+    L = np.array([[2.0, 0.0, 0.0], [0.3, 0.5, 0.0], [-1.0, 4.0, 3.0]])
+    assert orc.diag_product(L) == 3.0                        # 1 * 2 * 0.5 * 3
+    assert orc.diag_product(L) ** 2 == 9.0                   # Cholesky::determinant
+    assert abs(orc.chol_ln_determinant(L) - math.log(9.0)) < 1e-15
+    rng = np.random.default_rng(24)
+    G = rng.standard_normal((20, 20)); A = G @ G.T + 20 * np.eye(20)
+    Lf, info = orc.potrf(A)
+    assert info == 0
+    sign, logdet = np.linalg.slogdet(A)
+    assert sign == 1.0 and abs(orc.chol_ln_determinant(Lf) - logdet) < 1e-12 * abs(logdet)
+    assert abs(orc.diag_product(Lf) ** 2 / np.linalg.det(A) - 1.0) < 1e-12
+
+
 def test_qr_square_solve(orc):
     rng = np.random.default_rng(22)
     A = rng.uniform(-1, 1, (9, 9)); b = rng.uniform(-1, 1, (9, 2))
