@@ -90,21 +90,24 @@ def test_mixed_ir_short_last_block(nl, n, window):
     assert ok and be <= 1e-14 and berr(A, xv, b) <= 2e-14
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 128, 129, 200])
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 128, 129, 200, 255, 256, 257, 320, 383, 511, 512, 513])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_mixed_ir_window_edge_sizes(nl, n, prec):
     """Windowed pivoting at sizes around one block (64 columns), one window (128
-    rows) and the first look-ahead step, on well-conditioned matrices whose
-    largest entries sit off the diagonal (so the window has to pivot)."""
+    rows), the first look-ahead steps and up to eight blocks with a short
+    last one, on well-conditioned matrices whose largest entries sit off the
+    diagonal (so the window has to pivot).  The windowed factors must do
+    without the partial-pivoting fallback."""
     rng = np.random.default_rng(1000 + n)
     A = rng.standard_normal((n, n))
     A[np.arange(n), (np.arange(n) + 1) % n] += 4.0 * n ** 0.5  # dominant entries off the diagonal
     b = rng.standard_normal(n)
-    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32}[prec] | nl.PREC_WINDOW
+    p = {"fp64": nl.PREC_FP64, "fp32": nl.PREC_FP32, "bf16x3": nl.PREC_BF16X3}[prec] | nl.PREC_WINDOW
     x, iters, be, ok = nl.solve_mixed_ir(nl.DMatrix.from_numpy(A), nl.DMatrix.from_numpy(b.reshape(-1, 1)), p,
                                          tol=1e-14, maxit=20)
     xv = x.numpy()[:, 0]
     assert ok and be <= 1e-14, (iters, be)
+    assert not nl.last_ir_fallback()
     assert berr(A, xv, b) <= 2e-14
     assert np.allclose(xv, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
 
