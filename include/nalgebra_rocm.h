@@ -166,8 +166,8 @@ int nrocm_batch_set_graph(nrocm_batch batch, int enable);
  * the batch has no member of that size class). */
 int nrocm_batch_autotune(nrocm_handle handle, nrocm_batch batch, int mode, int *lanes);
 /* Schedule value of the element-tile kernel: one wavefront per matrix with the
- * lower triangle spread over an 8 x 8 grid of lanes (padded sizes 40..64 and
- * at most 8 right-hand sides); the autotuner times it beside 8..64 lanes. */
+ * lower triangle spread over an 8 x 8 grid of lanes (padded sizes 24..64 and
+ * at most 4 right-hand sides); the autotuner times it beside 8..64 lanes. */
 #define NROCM_SCHED_TILE 100
 /* Current lanes-per-matrix schedule of `mode` in the same format. */
 int nrocm_batch_schedule(nrocm_batch batch, int mode, int *lanes);

@@ -55,17 +55,18 @@ const FusedLaunch kFused[8] = {nrocm_chol_fused_launch_8,  nrocm_chol_fused_laun
 
 using TileLaunch = int (*)(bool, hipStream_t, const int *, int, int, const int *, double *const *, double *const *,
                           int, int *);
-// the element-tile schedule of padded sizes 40..64 (bins 4..7)
-const TileLaunch kTile[4] = {nrocm_chol_tile_launch_40, nrocm_chol_tile_launch_48, nrocm_chol_tile_launch_56,
+// the element-tile schedule of padded sizes 24..64 (bins 2..7)
+const TileLaunch kTile[6] = {nrocm_chol_tile_launch_24, nrocm_chol_tile_launch_32, nrocm_chol_tile_launch_40,
+                             nrocm_chol_tile_launch_48, nrocm_chol_tile_launch_56,
                              nrocm_chol_tile_launch_64};
 
-bool tile_ok(int bin, int nrhs) { return bin >= 4 && bin < 8 && nrhs <= nrocm_chol_tile::MAXRHS; }
+bool tile_ok(int bin, int nrhs) { return bin >= 2 && bin < 8 && nrhs <= nrocm_chol_tile::MAXRHS; }
 
 // One fused launch of a bin's members [begin, end) of `order` with schedule G.
 int launch_bin(bool exact, int G, int bin, hipStream_t st, const int *order, int begin, int end, const int *nsz,
                double *const *A, double *const *B, int nrhs, int *info)
 {
-    if (G == NROCM_SCHED_TILE) return kTile[bin - 4](exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    if (G == NROCM_SCHED_TILE) return kTile[bin - 2](exact, st, order, begin, end, nsz, A, B, nrhs, info);
     return kFused[bin](exact, G, st, order, begin, end, nsz, A, B, nrhs, info);
 }
 

@@ -1,4 +1,6 @@
 // Instantiates the batched Cholesky kernels for padded size 32.
 #include "chol_kernels.hpp"
+#include "chol_tile.hpp"
 
 NROCM_CHOL_DEFINE(32)
+NROCM_CHOL_TILE_DEFINE(32)

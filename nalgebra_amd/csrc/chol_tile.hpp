@@ -1,4 +1,4 @@
-// Batched Cholesky factor + solve for padded sizes 40..64 with one wavefront
+// Batched Cholesky factor + solve for padded sizes 24..64 with one wavefront
 // per matrix and the lower triangle spread over an 8 x 8 grid of lanes.
 //
 // Lane (p, q) = (lane % 8, lane / 8) holds the elements (p + 8 ri, q + 8 ci),
@@ -111,7 +111,7 @@ __device__ __forceinline__ double quotient(double v, double den, double y, bool 
 // Wavefronts per SIMD of each padded size (the tiles shrink with NB); at
 // NB = 64 the fourth wavefront costs 3 spilled vector registers.
 template <int NB>
-constexpr int tile_waves() { return NB >= 56 ? 4 : NB == 48 ? 5 : 6; }
+constexpr int tile_waves() { return NB >= 56 ? 4 : NB == 48 ? 5 : NB == 40 ? 6 : 8; }
 
 template <int NB, bool EXACT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tile_waves<NB>())))
@@ -356,6 +356,8 @@ int tile_launch(bool exact, hipStream_t st, const int *order, int begin, int end
         return nrocm_chol_tile::tile_launch<NB>(exact, st, order, begin, end, nsz, A, B, nrhs, info); \
     }
 
+NROCM_CHOL_TILE_DECLARE(24)
+NROCM_CHOL_TILE_DECLARE(32)
 NROCM_CHOL_TILE_DECLARE(40)
 NROCM_CHOL_TILE_DECLARE(48)
 NROCM_CHOL_TILE_DECLARE(56)

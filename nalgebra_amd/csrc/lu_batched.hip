@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
     int *const *__restrict__ Pptr, double *const *__restrict__ Bptr, int nrhs, int *__restrict__ info)
 {
     constexpr int MPW = 64 / G;
-    __shared__ double rowbuf[MPW][NB];
+    __shared__ __align__(16) double rowbuf[MPW][NB];
     __shared__ double bbuf[MPW][CH];
     __shared__ int where[MPW][NB];  // final position -> lane within the group
     const int lane = threadIdx.x;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
 #pragma unroll
             for (int k = i; k < NB; ++k) rowbuf[g][k] = a[k];
         }
-        __syncthreads();
+        lds_barrier();
         const double d = live ? rowbuf[g][i] : 1.0;
         if (live && r == 0 && P) P[i] = (d == 0.0) ? i : bp;
         if (live && d == 0.0) zmax = i + 1;  // 2. zero pivot: nothing moves
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
                 for (int k = i + 1; k < NB; ++k) a[k] = sub_prod<EXACT>(a[k], rowbuf[g][k], l);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     // the factors, each row at its final position
     if (row_ok) {
@@ -164,12 +164,12 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
 #pragma unroll
                 for (int q = 0; q < CH; ++q) bbuf[g][q] = b[q];
             }
-            __syncthreads();
+            lds_barrier();
             if (live && pos > i && row_ok) {
 #pragma unroll
                 for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
             }
-            __syncthreads();
+            lds_barrier();
         }
         // backward: x_i = b_i / U_ii, then the positions above it
 #pragma unroll
@@ -182,12 +182,12 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
                     bbuf[g][q] = b[q];
                 }
             }
-            __syncthreads();
+            lds_barrier();
             if (live && pos < i && row_ok) {
 #pragma unroll
                 for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
             }
-            __syncthreads();
+            lds_barrier();
         }
         if (solve && row_ok) {
 #pragma unroll

@@ -43,7 +43,7 @@ def test_batched_exact_is_bitwise(nl, orc, nrhs, sched):
     ns = NS_ALL
     print(f"batched exact: {len(ns)} members, nrhs={nrhs}, schedule {sched}", flush=True)
     sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
-           "tile": {nb: TILE for nb in range(40, 65, 8)} if nrhs <= 4 else None}[sched]
+           "tile": {nb: TILE for nb in range(24, 65, 8)} if nrhs <= 4 else None}[sched]
     A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 11, nl.EXACT, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
@@ -52,7 +52,7 @@ def test_batched_exact_is_bitwise(nl, orc, nrhs, sched):
     assert np.array_equal(_bits(Bg), _bits(Br)), "solution bits differ"
 
 
-TILE = 100  # NROCM_SCHED_TILE: one wavefront per matrix, 8 x 8 lane grid (padded sizes 40..64)
+TILE = 100  # NROCM_SCHED_TILE: one wavefront per matrix, 8 x 8 lane grid (padded sizes 24..64)
 
 
 @pytest.mark.parametrize("lanes", [8, 16, 32, 64, TILE])
@@ -64,7 +64,7 @@ def test_every_schedule_is_bitwise(nl, orc, lanes):
     A, offA, B, offB = gen.spd_batch(21, ns, 4)
     bt = nl.CholeskyBatch(ns, 4)
     if lanes == TILE:
-        want = {nb: TILE for nb in range(40, 65, 8)}
+        want = {nb: TILE for nb in range(24, 65, 8)}
     else:
         want = {nb: lanes for nb in range(8, 65, 8) if nb <= lanes}
     bt.set_schedule(want, nl.EXACT)
@@ -88,7 +88,7 @@ def test_autotune_keeps_parity(nl, orc):
     print("autotuned lanes per matrix:", chosen, flush=True)
     assert set(chosen) == {8 * ((n + 7) // 8) for n in ns}
     for nb, g in chosen.items():
-        assert (g in (8, 16, 32, 64) and g >= nb) or (g == TILE and nb >= 40)
+        assert (g in (8, 16, 32, 64) and g >= nb) or (g == TILE and nb >= 24)
     assert bt.schedule(nl.EXACT) == chosen
     Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
@@ -105,7 +105,7 @@ def test_autotune_keeps_parity(nl, orc):
 def test_batched_fast_within_1e12(nl, orc, sched):
     ns = NS_ALL
     sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
-           "tile": {nb: TILE for nb in range(40, 65, 8)}}[sched]
+           "tile": {nb: TILE for nb in range(24, 65, 8)}}[sched]
     A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
@@ -151,7 +151,7 @@ def test_batched_not_positive_definite(nl, orc, sched):
     L6 = np.linalg.cholesky(M6)
     M6[47, 47] -= L6[47, 47] ** 2  # the last pivot becomes (about) zero or negative
     bt = nl.CholeskyBatch(ns, 4)
-    bt.set_schedule({nb: (64 if sched == "rows" else TILE) for nb in range(40, 65, 8)}, nl.EXACT)
+    bt.set_schedule({nb: (64 if sched == "rows" else TILE) for nb in range(24, 65, 8)}, nl.EXACT)
     Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
     Ar, Br = A.copy(), B.copy()
