@@ -79,18 +79,36 @@ int default_lanes(int bin)
 
 int mpw_of_bin(int bin) { return 64 / default_lanes(bin); }
 
+// The bins are independent.  With `split`, the bins (sizes in decreasing
+// order) are dealt round-robin to the stream and kBranches - 1 auxiliary
+// streams, so that in a captured graph they form parallel branches whose
+// kernels run side by side (measured on the 65,536-member step: one branch
+// 1.134 ms, two 0.911, three 0.950, four 0.937).
+constexpr int kBranches = 2;
+static_assert(kBranches >= 1 && kBranches <= nrocm_handle_s::kAux + 1, "branches");
+
 int launch_batched(nrocm_handle h, nrocm_batch bt, hipStream_t st, double *const *A,
-                   double *const *B, int *info, bool exact)
+                   double *const *B, int *info, bool exact, bool split = false)
 {
+    split = split && kBranches > 1;
+    if (split) {
+        const int s = nrocm_fork_on(h, st, kBranches - 1);
+        if (s != NROCM_OK) return s;
+    }
+    int k = 0;
     for (const auto &bin : bt->bins) {
         const int items = bin.end - bin.begin;
         if (items <= 0) continue;
-        const int mpw = mpw_of_bin(bin.kernel);
-        (void)mpw;
         const int G = bt->lanes[exact ? 0 : 1][&bin - bt->bins.data()];
-        if (launch_bin(exact, G, bin.kernel, st, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
+        const int br = split ? k++ % kBranches : 0;
+        hipStream_t bs = br ? h->aux[br - 1] : st;
+        if (launch_bin(exact, G, bin.kernel, bs, bt->order_dev, bin.begin, bin.end, bt->n_dev, A,
                        bt->nrhs > 0 ? B : nullptr, bt->nrhs, info))
             NROCM_CHECK_LAUNCH(h);
+    }
+    if (split) {
+        const int s = nrocm_join_on(h, st, kBranches - 1);
+        if (s != NROCM_OK) return s;
     }
     return NROCM_OK;
 }
@@ -405,8 +423,10 @@ int nrocm_dpotrf_potrs_batched(nrocm_handle h, nrocm_batch bt, double *const *A,
         }
         if (!bt->graph_stream)
             NROCM_CHECK_HIP(h, hipStreamCreateWithFlags(&bt->graph_stream, hipStreamNonBlocking));
+        const int sa = nrocm_aux_init(h);  // streams and events exist before the capture
+        if (sa != NROCM_OK) return sa;
         NROCM_CHECK_HIP(h, hipStreamBeginCapture(bt->graph_stream, hipStreamCaptureModeThreadLocal));
-        int st = launch_batched(h, bt, bt->graph_stream, A, B, info, exact);
+        int st = launch_batched(h, bt, bt->graph_stream, A, B, info, exact, true);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(bt->graph_stream, &graph);
         if (st != NROCM_OK) {

@@ -425,13 +425,20 @@ extern "C" int nrocm_dgetrf_getrs_batched(nrocm_handle h, nrocm_batch bt, double
     if (mode != NROCM_EXACT && mode != NROCM_FAST) return -7;
     if (bt->count == 0) return NROCM_OK;
     if (bt->max_n > 64) return nrocm_fail(h, NROCM_ERR_UNSUPPORTED, "batched LU supports n <= 64");
+    // the bins are independent: dealt round-robin (sizes in decreasing order)
+    // to the stream and one auxiliary stream, whose kernels then run side by side
+    int s = nrocm_aux_init(h);
+    if (s == NROCM_OK) s = nrocm_fork_on(h, h->stream, 1);
+    if (s != NROCM_OK) return s;
+    int k = 0;
     for (const auto &bin : bt->bins) {
         if (bin.end <= bin.begin) continue;
-        if (kLaunch[bin.kernel](mode == NROCM_EXACT, h->stream, bt->order_dev, bin.begin, bin.end, bt->n_dev, a, ipiv,
+        hipStream_t bs = (k++ & 1) ? h->aux[0] : h->stream;
+        if (kLaunch[bin.kernel](mode == NROCM_EXACT, bs, bt->order_dev, bin.begin, bin.end, bt->n_dev, a, ipiv,
                                 bt->nrhs > 0 ? b : nullptr, bt->nrhs, info))
             NROCM_CHECK_LAUNCH(h);
     }
-    return NROCM_OK;
+    return nrocm_join_on(h, h->stream, 1);
 }
 
 extern "C" int nrocm_dgetrs_backward_batched(nrocm_handle h, nrocm_batch bt, const double *const *lu,
