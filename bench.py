@@ -554,13 +554,14 @@ def bench_adjoint(args):
     bt = nl.CholeskyBatch(ns, 1, device=dev)
     ptrs = [bt.pointers(a, b) for a, b in copies]
     infos = [torch.empty(count, dtype=torch.int32, device=dev) for _ in range(K)]
+    mode = nl.EXACT if args.mode == "exact" else nl.FAST
+    schedule = bt.autotune(mode)   # the forward's schedule, measured as in the batched Cholesky record
     dX = [torch.empty_like(B0) for _ in range(K)]
     dA = [torch.empty_like(A0) for _ in range(K)]
     dB = [torch.empty_like(B0) for _ in range(K)]
     from nalgebra_amd.batched import _pointer_array
     bptrs = [(ptrs[i][0], ptrs[i][1], _pointer_array(dX[i], bt.offB), _pointer_array(dA[i], bt.offA),
               _pointer_array(dB[i], bt.offB)) for i in range(K)]
-    mode = nl.EXACT if args.mode == "exact" else nl.FAST
 
     def step(i):
         bt.solve_ptrs(ptrs[i][0], ptrs[i][1], infos[i], mode)
@@ -594,7 +595,8 @@ def bench_adjoint(args):
     out = {"metric": METRIC, "value": count / (ms * 1e-3), "unit": "matrices/s (forward+backward)", "n_gpus": 1,
            "steps": K, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "cholesky_solve_adjoint", "count": count, "n": n, "mode": args.mode},
+           "config": {"workload": "cholesky_solve_adjoint", "count": count, "n": n, "mode": args.mode,
+                      "forward_lanes_per_matrix": {str(k): v for k, v in schedule.items()}},
            "roofline": {"bound": "hbm", "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                         "note": "the 54 MB working set is Infinity-Cache resident; launch/latency bound"},
