@@ -40,14 +40,16 @@ __device__ __forceinline__ double sub_prod(double acc, double x, double y)
     // is applied to x before the product, as there, so that a NaN coming from
     // y keeps its sign.  One asm block keeps the product next to its addition
     // (as two IR operations the scheduler hoists every product of a step and
-    // spills the row array).
-    double r;
+    // spills the row array), and it updates acc in place: with a separate
+    // output the elimination of a step writes a second copy of the row and
+    // moves it back at the end of the divergent region.
     if constexpr (EXACT) {
-        asm("v_mul_f64 %0, -%1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(r) : "v"(x), "v"(y), "v"(acc));
+        double t;
+        asm("v_mul_f64 %1, -%2, %3\n\tv_add_f64 %0, %1, %0" : "+v"(acc), "=&v"(t) : "v"(x), "v"(y));
     } else {
-        asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(acc));
+        asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(acc) : "v"(x), "v"(y));
     }
-    return r;
+    return acc;
 }
 
 // (value, position) under icamax's rule: larger value, then lower position.
@@ -124,6 +126,9 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
         const double d = live ? rowbuf[g][i] : 1.0;
         if (live && r == 0 && P) P[i] = (d == 0.0) ? i : bp;
         if (live && d == 0.0) zmax = i + 1;  // 2. zero pivot: nothing moves
+        // zmax settled now: left to the compiler, its update sinks below the
+        // loop and keeps every step's pivot value live to the end
+        asm volatile("" : "+v"(zmax));
         if (live && d != 0.0) {
             if (pos == i) pos = bp;
             else if (is_piv) pos = i;
