@@ -169,12 +169,17 @@ int nrocm_batch_autotune(nrocm_handle handle, nrocm_batch batch, int mode, int *
  * lower triangle spread over an 8 x 8 grid of lanes (padded sizes 24..64 and
  * at most 4 right-hand sides); the autotuner times it beside 8..64 lanes. */
 #define NROCM_SCHED_TILE 100
+/* Schedule value of the element-tile kernel with two matrices per wavefront,
+ * each over an 8 x 4 grid of 32 lanes (padded sizes 24 and 32, at most 4
+ * right-hand sides); bit-identical to NROCM_SCHED_TILE's results, and timed
+ * by the autotuner beside the other schedules. */
+#define NROCM_SCHED_TILE2 101
 /* Current lanes-per-matrix schedule of `mode` in the same format. */
 int nrocm_batch_schedule(nrocm_batch batch, int mode, int *lanes);
 /* Sets the schedule of `mode` (same format; 0 keeps a size class's current
  * choice).  Each entry must be 8, 16, 32 or 64 and at least the padded size,
- * or NROCM_SCHED_TILE where that kernel applies; otherwise -3 and nothing
- * changes. */
+ * or NROCM_SCHED_TILE / NROCM_SCHED_TILE2 where that kernel applies; otherwise
+ * -3 and nothing changes. */
 int nrocm_batch_set_schedule(nrocm_batch batch, int mode, const int *lanes);
 
 /* For each j: Cholesky::new(A_j) then solve_mut(B_j), in one persistent

@@ -16,6 +16,7 @@
 // and columns only ever meet zeros, and the dot lengths use the true n.
 #include "chol_kernels.hpp"
 #include "chol_tile.hpp"
+#include "chol_tile2.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -62,11 +63,25 @@ const TileLaunch kTile[6] = {nrocm_chol_tile_launch_24, nrocm_chol_tile_launch_3
 
 bool tile_ok(int bin, int nrhs) { return bin >= 2 && bin < 8 && nrhs <= nrocm_chol_tile::MAXRHS; }
 
+// the element-tile schedule with two matrices per wavefront, padded sizes 24 and 32 (bins 2, 3)
+const TileLaunch kTile2[2] = {nrocm_chol_tile2_launch_24, nrocm_chol_tile2_launch_32};
+
+bool tile2_ok(int bin, int nrhs) { return (bin == 2 || bin == 3) && nrhs <= nrocm_chol_tile::MAXRHS; }
+
+// Whether schedule G can run bin `bin` (padded size 8 (bin + 1)) with nrhs right-hand sides.
+bool schedule_ok(int G, int bin, int nrhs)
+{
+    if (G == NROCM_SCHED_TILE) return tile_ok(bin, nrhs);
+    if (G == NROCM_SCHED_TILE2) return tile2_ok(bin, nrhs);
+    return (G == 8 || G == 16 || G == 32 || G == 64) && G >= 8 * (bin + 1);
+}
+
 // One fused launch of a bin's members [begin, end) of `order` with schedule G.
 int launch_bin(bool exact, int G, int bin, hipStream_t st, const int *order, int begin, int end, const int *nsz,
                double *const *A, double *const *B, int nrhs, int *info)
 {
     if (G == NROCM_SCHED_TILE) return kTile[bin - 2](exact, st, order, begin, end, nsz, A, B, nrhs, info);
+    if (G == NROCM_SCHED_TILE2) return kTile2[bin - 2](exact, st, order, begin, end, nsz, A, B, nrhs, info);
     return kFused[bin](exact, G, st, order, begin, end, nsz, A, B, nrhs, info);
 }
 
@@ -159,10 +174,8 @@ int autotune(nrocm_handle h, nrocm_batch bt, bool exact)
     int status = NROCM_OK;
     for (size_t bi = 0; bi < bt->bins.size() && status == NROCM_OK; ++bi) {
         const auto &bin = bt->bins[bi];
-        const int cand[5] = {8, 16, 32, 64, NROCM_SCHED_TILE};
-        auto usable = [&](int G) {
-            return G == NROCM_SCHED_TILE ? tile_ok(bin.kernel, bt->nrhs) : G >= bin.nb;
-        };
+        const int cand[6] = {8, 16, 32, 64, NROCM_SCHED_TILE, NROCM_SCHED_TILE2};
+        auto usable = [&](int G) { return schedule_ok(G, bin.kernel, bt->nrhs); };
         int ncand = 0;
         for (int G : cand) ncand += usable(G) ? 1 : 0;
         if (ncand < 2) continue;
@@ -370,11 +383,7 @@ int nrocm_batch_set_schedule(nrocm_batch bt, int mode, const int *lanes)
         return nrocm_fail(bt->handle, NROCM_ERR_UNSUPPORTED, "batched Cholesky supports n <= 64");
     for (int c = 0; c < 8; ++c) {
         const int G = lanes[c];
-        if (G == NROCM_SCHED_TILE) {
-            if (!tile_ok(c, bt->nrhs)) return -3;
-        } else if (G != 0 && ((G != 8 && G != 16 && G != 32 && G != 64) || G < 8 * (c + 1))) {
-            return -3;
-        }
+        if (G != 0 && !schedule_ok(G, c, bt->nrhs)) return -3;
     }
     const int mi = mode == NROCM_EXACT ? 0 : 1;
     for (size_t bi = 0; bi < bt->bins.size(); ++bi)

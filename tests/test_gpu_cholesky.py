@@ -38,12 +38,13 @@ def _run_batch(nl, ns, nrhs, seed, mode, graph=True, schedule=None):
 
 
 @pytest.mark.parametrize("nrhs", [4, 1, 6, 0, 3])
-@pytest.mark.parametrize("sched", ["default", "rows", "tile"])
+@pytest.mark.parametrize("sched", ["default", "rows", "tile", "tile2"])
 def test_batched_exact_is_bitwise(nl, orc, nrhs, sched):
     ns = NS_ALL
     print(f"batched exact: {len(ns)} members, nrhs={nrhs}, schedule {sched}", flush=True)
     sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
-           "tile": {nb: TILE for nb in range(24, 65, 8)} if nrhs <= 4 else None}[sched]
+           "tile": {nb: TILE for nb in range(24, 65, 8)} if nrhs <= 4 else None,
+           "tile2": {24: TILE2, 32: TILE2} if nrhs <= 4 else None}[sched]
     A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, nrhs, 11, nl.EXACT, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
@@ -53,9 +54,10 @@ def test_batched_exact_is_bitwise(nl, orc, nrhs, sched):
 
 
 TILE = 100  # NROCM_SCHED_TILE: one wavefront per matrix, 8 x 8 lane grid (padded sizes 24..64)
+TILE2 = 101  # NROCM_SCHED_TILE2: two matrices per wavefront, 8 x 4 lanes each (padded sizes 24, 32)
 
 
-@pytest.mark.parametrize("lanes", [8, 16, 32, 64, TILE])
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64, TILE, TILE2])
 def test_every_schedule_is_bitwise(nl, orc, lanes):
     """Each compiled schedule (the autotuner may pick any) reproduces the
     oracle bit for bit; size classes that do not fit `lanes` keep their
@@ -65,6 +67,8 @@ def test_every_schedule_is_bitwise(nl, orc, lanes):
     bt = nl.CholeskyBatch(ns, 4)
     if lanes == TILE:
         want = {nb: TILE for nb in range(24, 65, 8)}
+    elif lanes == TILE2:
+        want = {24: TILE2, 32: TILE2}
     else:
         want = {nb: lanes for nb in range(8, 65, 8) if nb <= lanes}
     bt.set_schedule(want, nl.EXACT)
@@ -88,7 +92,7 @@ def test_autotune_keeps_parity(nl, orc):
     print("autotuned lanes per matrix:", chosen, flush=True)
     assert set(chosen) == {8 * ((n + 7) // 8) for n in ns}
     for nb, g in chosen.items():
-        assert (g in (8, 16, 32, 64) and g >= nb) or (g == TILE and nb >= 24)
+        assert (g in (8, 16, 32, 64) and g >= nb) or (g == TILE and nb >= 24) or (g == TILE2 and nb in (24, 32))
     assert bt.schedule(nl.EXACT) == chosen
     Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
     info = bt.solve(Ad, Bd, nl.EXACT).cpu().numpy()
@@ -101,11 +105,12 @@ def test_autotune_keeps_parity(nl, orc):
         bt.set_schedule({64: 32}, nl.EXACT)
 
 
-@pytest.mark.parametrize("sched", ["default", "rows", "tile"])
+@pytest.mark.parametrize("sched", ["default", "rows", "tile", "tile2"])
 def test_batched_fast_within_1e12(nl, orc, sched):
     ns = NS_ALL
     sch = {"default": None, "rows": {nb: 64 for nb in range(40, 65, 8)},
-           "tile": {nb: TILE for nb in range(24, 65, 8)}}[sched]
+           "tile": {nb: TILE for nb in range(24, 65, 8)},
+           "tile2": {24: TILE2, 32: TILE2}}[sched]
     A, offA, B, offB, Ag, Bg, info = _run_batch(nl, ns, 4, 12, nl.FAST, schedule=sch)
     Ar, Br = A.copy(), B.copy()
     orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, 4)
@@ -164,6 +169,52 @@ def test_batched_not_positive_definite(nl, orc, sched):
     for j in [j for j in range(len(ns)) if info_r[j] == 0]:
         sl = slice(offB[j], offB[j] + 4 * ns[j])
         assert np.array_equal(_bits(Bg[sl]), _bits(Br[sl]))
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("nrhs", [4, 2, 0])
+def test_tile2_members_pair_independently(nl, orc, mode, nrhs):
+    """Two matrices share a wavefront in NROCM_SCHED_TILE2: pairs of different
+    sizes, a failing member beside a healthy one, and an odd member count (the
+    last wavefront holds one member).  EXACT matches the oracle bit for bit,
+    FAST matches NROCM_SCHED_TILE bit for bit (the two run the same operations
+    per element)."""
+    ns = [20, 30, 25, 17, 32, 24, 31, 18, 29]
+    A, offA, B, offB = gen.spd_batch(31, ns, nrhs)
+    # member 1: NaN diagonal; member 2: negative pivot at column 3;
+    # member 4: NaN below the diagonal in its last column
+    A[offA[1]] = np.nan
+    M2 = A[offA[2]:offA[2] + 625].reshape(25, 25, order="F")
+    M2[3, 3] = -50.0
+    A[offA[4] + 31 + 30 * 32] = np.nan
+    m = nl.EXACT if mode == "exact" else nl.FAST
+    out = {}
+    for sched in (TILE, TILE2):
+        bt = nl.CholeskyBatch(ns, nrhs)
+        bt.set_schedule({24: sched, 32: sched}, m)
+        assert bt.schedule(m)[24] == sched and bt.schedule(m)[32] == sched
+        Ad = torch.from_numpy(A).cuda(); Bd = torch.from_numpy(B).cuda()
+        info = bt.solve(Ad, Bd, m).cpu().numpy()
+        out[sched] = (info, Ad.cpu().numpy(), Bd.cpu().numpy())
+    info2, A2, B2 = out[TILE2]
+    assert np.array_equal(out[TILE][0], info2)
+    assert np.array_equal(_bits(out[TILE][2]), _bits(B2)), "TILE2 solutions differ from TILE"
+    Ar, Br = A.copy(), B.copy()
+    info_r = orc.potrf_potrs_batched(ns, Ar, offA, Br, offB, nrhs)
+    assert list(info2) == list(info_r)
+    assert info_r[1] == 1 and info_r[2] == 4 and info_r[4] == 32 and info_r[0] == 0
+    for j in range(len(ns)):
+        sl = slice(offB[j], offB[j] + nrhs * ns[j])
+        if info_r[j] != 0:
+            assert np.array_equal(_bits(B2[sl]), _bits(B[sl])), f"failed member {j} changed"
+        elif mode == "exact":
+            assert np.array_equal(_bits(B2[sl]), _bits(Br[sl])), f"member {j}"
+            sa = slice(offA[j], offA[j] + ns[j] * ns[j])
+            L2 = np.tril(A2[sa].reshape(ns[j], ns[j], order="F"))
+            Lr = np.tril(Ar[sa].reshape(ns[j], ns[j], order="F"))
+            assert np.array_equal(_bits(L2), _bits(Lr)), f"factor of member {j}"
+        else:
+            assert _rel(B2[sl], Br[sl]) <= 1e-12, j
 
 
 def test_batched_graph_replays_match_eager(nl):

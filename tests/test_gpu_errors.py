@@ -166,7 +166,12 @@ def test_batch_argument_errors(env):
     lanes[0] = 0
     lanes[1] = 100  # NROCM_SCHED_TILE exists for padded sizes 24..64 only
     assert L.nrocm_batch_set_schedule(bt, EXACT, lanes) == -3
+    lanes[1] = 101  # NROCM_SCHED_TILE2 for padded sizes 24 and 32 only
+    assert L.nrocm_batch_set_schedule(bt, EXACT, lanes) == -3
     lanes[1] = 0
+    lanes[4] = 101
+    assert L.nrocm_batch_set_schedule(bt, EXACT, lanes) == -3
+    lanes[4] = 0
     assert L.nrocm_batch_schedule(bt, EXACT, None) == -3
     assert L.nrocm_batch_autotune(h, bt, 2, lanes) == -3
     assert L.nrocm_batch_destroy(bt) == 0
