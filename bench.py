@@ -694,8 +694,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (batched members shard "
-              "across GPUs without communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (batched members would "
+              "shard across GPUs by member range without communication). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "all":
         # the default line: the batched-Cholesky record (headline value), with
