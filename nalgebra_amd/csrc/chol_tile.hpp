@@ -58,6 +58,17 @@ __device__ const double kZeroCellT[2] = {0.0, 1.0};  // a zero, then a one for p
 // barrier drains: each step would wait for its own column store).
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) const char gcchar;
+
+// Element at a 32-bit byte offset from a wavefront-uniform base: the scalar-
+// base addressing form (one vector add per access instead of a 64-bit address
+// per lane).  A member's matrix (n <= 64) spans far less than 4 GiB.  (Used
+// for the backward's loads; the same form for the factor's column stores
+// costs the padded-64 kernel 16 more spilled registers.)
+__device__ __forceinline__ double gload_at(gcdouble *base, unsigned byte_off)
+{
+    return *(gcdouble *)((gcchar *)base + byte_off);
+}
 
 // 64-bit DPP row shift left by N within rows of 16 lanes; lanes whose source
 // leaves the row get an unspecified value (their results are not used).
@@ -259,8 +270,7 @@ void potrf_potrs_tile_kernel(const int *__restrict__ order, int begin, int end, 
     constexpr int PF = 8;
     const int rl = row_ok ? r : n - 1;
     auto lval = [&](int i) {
-        gcdouble *col = A + (size_t)(i > 0 ? i : 0) * lda;  // wavefront-uniform
-        return col[rl];
+        return gload_at(A, (unsigned)((i > 0 ? i : 0) * lda + rl) * 8u);
     };
     double cur[PF], nxt[PF];
 #pragma unroll
