@@ -58,17 +58,59 @@ __device__ __forceinline__ bool beats(double av, int ap, double bv, int bp)
     return av > bv || (av == bv && ap < bp);
 }
 
-// Group-wide pivot: every lane of the group ends with the winner.
+// One DPP step of the pivot scan: (v, pos) against the pair the pattern CTRL
+// brings in (rows outside ROWMASK, and lanes without a source, see the
+// non-candidate pair (-1, INT_MAX), which never wins).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void pivot_dpp_step(double &v, int &pos)
+{
+    const long long x = __double_as_longlong(v);
+    const long long o = __double_as_longlong(-1.0);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWMASK, 0xf, false);
+    const double ov = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    const int op = __builtin_amdgcn_update_dpp(0x7fffffff, pos, CTRL, ROWMASK, 0xf, false);
+    if (beats(ov, op, v, pos)) {
+        v = ov;
+        pos = op;
+    }
+}
+
+// Group-wide pivot: every lane of the group ends with the winner's position.
+// Groups of 16 or more lanes reduce by DPP scans (the rule is a total order,
+// so the winner does not depend on the combination order): row_shr 1, 2, 4, 8
+// leave each 16-lane row's winner in its lane 15; row_bcast 15 into rows 1
+// and 3 only joins the rows of each 32-lane half, and row_bcast 31 the two
+// halves.  Groups of 8 use a butterfly of lane shuffles (row_shr 4 would
+// cross their boundary).
 template <int G>
 __device__ __forceinline__ void group_argmax(double &v, int &pos)
 {
+    if constexpr (G >= 16) {
+        pivot_dpp_step<0x111, 0xf>(v, pos);
+        pivot_dpp_step<0x112, 0xf>(v, pos);
+        pivot_dpp_step<0x114, 0xf>(v, pos);
+        pivot_dpp_step<0x118, 0xf>(v, pos);
+        if constexpr (G >= 32) pivot_dpp_step<0x142, 0xa>(v, pos);
+        if constexpr (G == 64) pivot_dpp_step<0x143, 0xc>(v, pos);
+        const int lane = threadIdx.x & 63;
+        if constexpr (G == 64) {
+            pos = __builtin_amdgcn_readlane(pos, 63);
+        } else if constexpr (G == 32) {
+            const int p0 = __builtin_amdgcn_readlane(pos, 31), p1 = __builtin_amdgcn_readlane(pos, 63);
+            pos = lane < 32 ? p0 : p1;
+        } else {
+            pos = __shfl(pos, lane | 15);
+        }
+    } else {
 #pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(v, off, G);
-        const int op = __shfl_xor(pos, off, G);
-        if (beats(ov, op, v, pos)) {
-            v = ov;
-            pos = op;
+        for (int off = G / 2; off >= 1; off >>= 1) {
+            const double ov = __shfl_xor(v, off, G);
+            const int op = __shfl_xor(pos, off, G);
+            if (beats(ov, op, v, pos)) {
+                v = ov;
+                pos = op;
+            }
         }
     }
 }
