@@ -117,6 +117,15 @@ __device__ __forceinline__ void group_argmax(double &v, int &pos)
 
 __device__ const double kZeroCellLU[1] = {0.0};
 
+// v of lane src (wavefront-uniform) in every lane.
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int NB, int G, bool EXACT>
 __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
     const int *__restrict__ order, int begin, int end, const int *__restrict__ nsz, double *const *__restrict__ Aptr,
@@ -203,38 +212,69 @@ __global__ __launch_bounds__(64) void getrf_getrs_batched_kernel(
         double b[CH];
 #pragma unroll
         for (int q = 0; q < CH; ++q) b[q] = (solve && row_ok && c0 + q < nrhs) ? B[r + (size_t)(c0 + q) * n] : 0.0;
-        // forward: unit lower, per element the steps i = 0, 1, ... in order
+        // forward: unit lower, per element the steps i = 0, 1, ... in order.
+        // A whole-wavefront group reads position i's values straight from the
+        // lane holding it (a wavefront-uniform lane); smaller groups pass them
+        // through LDS.
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             const bool live = i + 1 < n;
-            if (live && where[g][i] == r) {
+            double bv[CH];
+            if constexpr (G == 64) {
+                const int src = __builtin_amdgcn_readfirstlane(live ? where[0][i] : 0);
 #pragma unroll
-                for (int q = 0; q < CH; ++q) bbuf[g][q] = b[q];
+                for (int q = 0; q < CH; ++q) bv[q] = lane_value(b[q], src);
+            } else {
+                if (live && where[g][i] == r) {
+#pragma unroll
+                    for (int q = 0; q < CH; ++q) bbuf[g][q] = b[q];
+                }
+                lds_barrier();
+#pragma unroll
+                for (int q = 0; q < CH; ++q) bv[q] = bbuf[g][q];
             }
-            lds_barrier();
             if (live && pos > i && row_ok) {
 #pragma unroll
-                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
+                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bv[q], a[i]);
             }
-            lds_barrier();
+            if constexpr (G != 64) lds_barrier();
+            // one step at a time: interleaved across the unrolled steps, the
+            // broadcasts and index reads run out of registers
+            if constexpr (G == 64) __builtin_amdgcn_sched_barrier(0);
         }
         // backward: x_i = b_i / U_ii, then the positions above it
 #pragma unroll
         for (int i = NB - 1; i >= 0; --i) {
             const bool live = i < n;
-            if (live && where[g][i] == r) {
+            double bv[CH];
+            if constexpr (G == 64) {
+                const int src = __builtin_amdgcn_readfirstlane(live ? where[0][i] : 0);
+                if (live && src == r) {
 #pragma unroll
-                for (int q = 0; q < CH; ++q) {
-                    b[q] = b[q] / a[i];
-                    bbuf[g][q] = b[q];
+                    for (int q = 0; q < CH; ++q) b[q] = b[q] / a[i];
                 }
+#pragma unroll
+                for (int q = 0; q < CH; ++q) bv[q] = lane_value(b[q], src);
+            } else {
+                if (live && where[g][i] == r) {
+#pragma unroll
+                    for (int q = 0; q < CH; ++q) {
+                        b[q] = b[q] / a[i];
+                        bbuf[g][q] = b[q];
+                    }
+                }
+                lds_barrier();
+#pragma unroll
+                for (int q = 0; q < CH; ++q) bv[q] = bbuf[g][q];
             }
-            lds_barrier();
             if (live && pos < i && row_ok) {
 #pragma unroll
-                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bbuf[g][q], a[i]);
+                for (int q = 0; q < CH; ++q) b[q] = sub_prod<EXACT>(b[q], bv[q], a[i]);
             }
-            lds_barrier();
+            if constexpr (G != 64) lds_barrier();
+            // one step at a time: interleaved across the unrolled steps, the
+            // broadcasts and index reads run out of registers
+            if constexpr (G == 64) __builtin_amdgcn_sched_barrier(0);
         }
         if (solve && row_ok) {
 #pragma unroll
